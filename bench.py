@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Benchmark of the RSF level-set evolution on one MI355X.
+
+Headline workload (the plain drop-in configuration): a 512^3 uint8 tube
+phantom (300 tubes, radius 2-6, +-20 x ramp, noise sigma 15, seed 2), fp32 phi0
+= signed distance to the thresholded sigma-1 blur, sigma = 3 (21 taps),
+lambda1 = lambda2 = 1, nu = 65.025, mu = 1, dt = 0.1, epsilon = 1, 500
+iterations.  A step is one RSF iteration over the whole volume; `value` is
+voxel-iterations per second with image and phi resident in HBM.
+
+Also reported on the same line:
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                mean launch time (HIP events on the launch stream), against
+                MI355X's 8 TB/s; `traffic` = rocprofv3-measured bytes/launch
+                from profiles/ when present;
+  loop_roofline 21 B per voxel-iteration over the whole timed loop;
+  batched       64 uneven blocks (sides from {96,160,192,256}, seed 1000+b),
+                300 iterations: one persistent launch per sweep per
+                iteration in a graph, vs the same blocks one after another;
+  cpu_baseline  the oracle (CPU restatement of the reference pipeline,
+                unfused, FP32) on the same 512^3 volume for a bounded number
+                of iterations on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "paper-rsf-vesselseg_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "Gvoxel-iterations/s + % HBM roofline; batched vs plain per-block, same GPU"
+HBM_PEAK_GBS = 8000.0
+SWEEP_BYTES = {"sweep1": 9, "sweep2": 12}   # algorithmic bytes per voxel per launch
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def params(rsf_amd, iterations, mode):
+    return rsf_amd.make_params(sigma=3.0, lambda1=1.0, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0,
+                               iterations=iterations, mode=mode)
+
+
+def plain_run(rsf_amd, torch, ctx, n, steps, warmup, mode, tune):
+    img, _ = rsf_amd.synth_tubes((n, n, n), tubes=300, rmin=2.0, rmax=6.0, bias=20.0, noise=15.0, seed=2)
+    phi0 = rsf_amd.initial_phi(img, 1.0, 110.0)
+    log(f"generated {n}^3 phantom, {float((phi0 < 0).mean()):.3f} inside")
+    dev = torch.device("cuda:0")
+    d_img = torch.from_numpy(img).to(dev)
+    d_phi0 = torch.from_numpy(phi0).to(dev)
+    d_phi = d_phi0.clone()
+    blocks = [(0, n, n, n)]
+    choice = None
+    if tune:
+        choice = ctx.autotune(blocks, rsf_amd.DTYPE_U8, params(rsf_amd, 1, mode))
+        log(f"autotune: {choice}")
+    if warmup > 0:
+        ctx.run(d_img, d_phi, params(rsf_amd, warmup, mode))
+    pk = params(rsf_amd, steps, mode)
+    d_phi.copy_(d_phi0)
+    ctx.run(d_img, d_phi, pk)          # captures and caches the graph of this loop
+    d_phi.copy_(d_phi0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loop_ms = ctx.run(d_img, d_phi, pk)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    vox = float(n) ** 3
+    inside = float((d_phi < 0).float().mean().item())
+    log(f"plain {n}^3 x {steps}: {wall * 1e3:.1f} ms wall, {loop_ms:.1f} ms loop")
+    # per-kernel timing, eager launches with events around each sweep
+    d_phi.copy_(d_phi0)
+    ms1, ms2 = ctx.profile_sweeps(blocks, d_img, d_phi, params(rsf_amd, 20, mode))
+    log(f"sweep1 {ms1:.3f} ms, sweep2 {ms2:.3f} ms per launch")
+    return dict(wall=wall, loop_ms=loop_ms, vox=vox, ms1=ms1, ms2=ms2, choice=choice, img=img, phi0=phi0,
+                inside=inside)
+
+
+def batched_run(rsf_amd, torch, ctx, steps, mode, nblocks=64):
+    import numpy as np
+
+    rng = np.random.default_rng(3)
+    sides = [96, 160, 192, 256]
+    shapes = [tuple(int(s) for s in rng.choice(sides, 3)) for _ in range(nblocks)]
+    blocks, total = rsf_amd.pack_blocks(shapes)
+    img = np.empty(total, np.uint8)
+    phi = np.empty(total, np.float32)
+    for b, ((off, w, h, l), shp) in enumerate(zip(blocks, shapes)):
+        nv = w * h * l
+        tubes = max(4, int(round(300 * nv / 512 ** 3)))
+        bi, _ = rsf_amd.synth_tubes(shp, tubes=tubes, rmin=2.0, rmax=6.0, bias=20.0, noise=15.0, seed=1000 + b)
+        img[off:off + nv] = bi.ravel()
+        phi[off:off + nv] = rsf_amd.initial_phi(bi, 1.0, 110.0).ravel()
+    log(f"generated {nblocks} blocks, {total / 1e6:.1f} Mvoxel")
+    dev = torch.device("cuda:0")
+    d_img = torch.from_numpy(img).to(dev)
+    d_phi0 = torch.from_numpy(phi).to(dev)
+    p = params(rsf_amd, steps, mode)
+    out = {}
+    for name, fn in (("batched", ctx.run_batched), ("per_block", ctx.run_sequential)):
+        d_phi = d_phi0.clone()
+        fn(blocks, d_img, d_phi, p)          # capture graphs
+        d_phi.copy_(d_phi0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(blocks, d_img, d_phi, p)
+        torch.cuda.synchronize()
+        out[name] = time.perf_counter() - t0
+        out[name + "_phi"] = d_phi
+        log(f"{name}: {out[name] * 1e3:.1f} ms")
+    same = bool(torch.equal(out["batched_phi"], out["per_block_phi"]))
+    gv = lambda t: total * steps / t / 1e9  # noqa: E731
+    return {
+        "blocks": nblocks,
+        "voxels": int(total),
+        "iterations": steps,
+        "value": round(gv(out["batched"]), 3),
+        "plain_per_block": round(gv(out["per_block"]), 3),
+        "speedup": round(out["per_block"] / out["batched"], 3),
+        "results_identical": same,
+    }
+
+
+def cpu_baseline(img, phi0, budget_s):
+    import numpy as np
+
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    p = oracle.make_params(sigma=3.0, lambda1=1.0, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0)
+    I = img.astype(np.float32)
+    t0 = time.perf_counter()
+    phi = oracle.run(phi0, I, p, 1, threads)
+    t1 = time.perf_counter() - t0
+    iters = 1
+    extra = int(max(0, min(8, budget_s / max(t1, 1e-3) - 1)))
+    if extra:
+        t0 = time.perf_counter()
+        oracle.run(phi, I, p, extra, threads)
+        t1 += time.perf_counter() - t0
+        iters += extra
+    n = phi0.size
+    log(f"cpu oracle: {iters} iterations of {n / 1e6:.0f} Mvoxel in {t1:.1f} s on {threads} threads")
+    return {
+        "value": round(n * iters / t1 / 1e9, 5),
+        "unit": "Gvoxel-iterations/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{iters} iteration(s) of the same {round(n ** (1 / 3))}^3 workload (oracle: unfused FP32 "
+                  f"restatement of CUDA_RSF/main.cpp:234-300, OpenMP)",
+    }
+
+
+def traffic_from_profiles(kernel):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=512, help="cube side of the plain workload")
+    ap.add_argument("--mode", choices=["fast", "literal"], default="fast")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=300)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the RSF engine runs each volume or batch of blocks on a single "
+              "device; multi-GPU sharding is a design note only (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import rsf_amd
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    rsf_amd.lib()
+    mode = rsf_amd.MODE_FAST if args.mode == "fast" else rsf_amd.MODE_LITERAL
+    ctx = rsf_amd.Context(0)
+    r = plain_run(rsf_amd, torch, ctx, args.size, args.steps, args.warmup, mode, not args.no_tune)
+    N, K = r["vox"], args.steps
+    value = N * K / r["wall"] / 1e9
+    dom = "sweep1" if r["ms1"] >= r["ms2"] else "sweep2"
+    ms_dom = r["ms1"] if dom == "sweep1" else r["ms2"]
+    achieved = SWEEP_BYTES[dom] * N / (ms_dom * 1e-3) / 1e9
+    traffic = traffic_from_profiles(dom)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gvoxel-iterations/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["wall"] * 1e3 / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain RSF drop-in: {args.size}^3 uint8 tube phantom (300 tubes, seed 2), fp32 phi, "
+                        f"sigma 3 (21 taps), lambda 1, nu 65.025, mu 1, dt 0.1, eps 1, {K} iterations",
+            "volume": [args.size] * 3,
+            "iterations": K,
+            "mode": args.mode,
+            "kernels": r["choice"],
+            "loop_ms_events": round(r["loop_ms"], 3),
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_voxel": SWEEP_BYTES[dom],
+            "ms_per_launch": {"sweep1": round(r["ms1"], 4), "sweep2": round(r["ms2"], 4)},
+        },
+        "loop_roofline": {
+            "bytes_per_voxel_iteration": 21,
+            "achieved": round(21 * N * K / r["wall"] / 1e9, 1),
+            "frac": round(21 * N * K / r["wall"] / 1e9 / HBM_PEAK_GBS, 4),
+        },
+    }
+    if not args.no_batched:
+        line["batched"] = batched_run(rsf_amd, torch, ctx, args.batched_steps, mode)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(r["img"], r["phi0"], args.cpu_budget)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,937 @@
+// Host engine behind the C ABI: contexts, workspace, the graph-captured
+// iteration loop (CUDA_RSF/main.cpp:222-314), batched and per-block runs, the
+// autotuner and the unfused device pipeline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <list>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "rsf_internal.hpp"
+
+namespace rsf {
+
+// ------------------------------------------------------------ error plumbing
+static thread_local std::string g_last_error;
+
+int set_error(int status, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    int st = (e == hipErrorOutOfMemory) ? RSF_ERR_OUT_OF_MEMORY : RSF_ERR_DEVICE;
+    return set_error(st, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+}
+
+int DevBuf::ensure(size_t n) {
+    if (n <= bytes && p) return RSF_OK;
+    release();
+    if (n == 0) return RSF_OK;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        return set_error(RSF_ERR_OUT_OF_MEMORY, "hipMalloc(%zu bytes): %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+    return RSF_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// --------------------------------------------------------------- parameters
+static int taps32(float sigma, float* g, int cap, int* P_out, int* m_out) {
+    // main.cpp:152-166 (taps), :264,273 (pad).
+    if (!(sigma > 0.0f) || !std::isfinite(sigma))
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "sigma must be positive, got %g", sigma);
+    const float s7 = 7 * sigma;
+    const unsigned int p = (unsigned int)s7;
+    const int k_size = (int)s7;
+    if (p % 2 == 0)
+        return set_error(RSF_ERR_INVALID_ARGUMENT,
+                         "floor(7*sigma) = %u is even: the reference writes its blurred buffers with "
+                         "%u taps but sizes them for %u (main.cpp:121,152-155,191); choose sigma with "
+                         "an odd floor(7*sigma)", p, p + 1, p);
+    if ((int)p > cap) return set_error(RSF_ERR_UNSUPPORTED, "sigma %g needs %u taps (max %d)", sigma, p, cap);
+    const float miu = (float)(p / 2);
+    const int u = (int)(2 * sigma * sigma);
+    if (u <= 0) return set_error(RSF_ERR_INVALID_ARGUMENT, "sigma %g gives a zero tap width (2 sigma^2 < 1)", sigma);
+    for (int xi = 0; xi < (int)p; xi++) {
+        const float d = (float)xi - miu;
+        const float ex = (-d) * d / (float)u;
+        g[xi] = 1.0f / sqrtf((float)u * (float)kPi) * expf(ex);
+    }
+    *P_out = (int)p;
+    *m_out = (k_size - 1) / 2;
+    return RSF_OK;
+}
+
+int check_params(const rsf_params* p) {
+    if (!p) return set_error(RSF_ERR_INVALID_ARGUMENT, "params is NULL");
+    const float v[] = {p->sigma, p->lambda1, p->lambda2, p->nu, p->mu, p->dt, p->epsilon, p->sigma2};
+    for (float x : v)
+        if (!std::isfinite(x)) return set_error(RSF_ERR_INVALID_ARGUMENT, "parameters must be finite");
+    if (!(p->epsilon > 0)) return set_error(RSF_ERR_INVALID_ARGUMENT, "epsilon must be > 0");
+    if (p->iterations < 0) return set_error(RSF_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+    if (p->mode != RSF_MODE_FAST && p->mode != RSF_MODE_LITERAL)
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "unknown mode %d", p->mode);
+    if ((int)p->sigma2 < 0) return set_error(RSF_ERR_INVALID_ARGUMENT, "sigma2 must be >= 0");
+    return RSF_OK;
+}
+
+int derive(const rsf_params* p, Derived* d) {
+    RSF_TRY(check_params(p));
+    RSF_TRY(taps32(p->sigma, d->taps.g, kMaxTaps, &d->P, &d->m));
+    double s = 0;
+    for (int k = 0; k < d->P; ++k) s += d->taps.g[k];
+    d->halfS3 = (float)(0.5 * s * s * s);
+    return RSF_OK;
+}
+
+}  // namespace rsf
+
+using namespace rsf;
+
+// ------------------------------------------------------------------- context
+namespace {
+
+struct GraphSet {
+    hipGraphExec_t main = nullptr, rem = nullptr;
+    int n_main = 0, n_launch = 0, n_rem = 0;
+    void reset() {
+        if (main) (void)hipGraphExecDestroy(main);
+        if (rem) (void)hipGraphExecDestroy(rem);
+        main = rem = nullptr;
+        n_main = n_launch = n_rem = 0;
+    }
+};
+
+struct Runner {
+    std::string key;
+    TilePlan plan;
+    GraphSet graphs;
+    std::string graph_key;
+    uint64_t ws_epoch = 0;
+    ~Runner() { graphs.reset(); }
+};
+
+struct TuneChoice {
+    std::string s1, s2;
+    int zc1 = 0, zc2 = 0;
+};
+
+}  // namespace
+
+struct rsf_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int num_cus = 256;
+    // workspace (grow-only); ws_epoch changes whenever a pointer changes
+    DevBuf phi2, tr, F, blurI, imgf, tmp1, tmp2, fo, fi, Eo, Ei;
+    DevBuf unf[20];
+    DevBuf h_img, h_phi;  // staging of rsf_run_host
+    uint64_t ws_epoch = 1;
+    std::list<std::unique_ptr<Runner>> runners;
+    std::map<std::string, TuneChoice> tune;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> iter_events;
+    float last_ms = 0.f;
+};
+
+namespace {
+
+int ensure_ws(rsf_context* c, DevBuf& b, size_t bytes) {
+    if (b.p && b.bytes >= bytes) return RSF_OK;
+    RSF_TRY(b.ensure(bytes));
+    c->ws_epoch++;
+    return RSF_OK;
+}
+
+std::string blocks_key(const std::vector<BlockDesc>& b) {
+    std::ostringstream os;
+    for (auto& x : b) os << x.off << ':' << x.w << 'x' << x.h << 'x' << x.l << ';';
+    return os.str();
+}
+
+std::string tune_key(const std::vector<BlockDesc>& b, int P, int literal) {
+    std::vector<std::string> dims;
+    for (auto& x : b) {
+        std::ostringstream os;
+        os << x.w << 'x' << x.h << 'x' << x.l;
+        dims.push_back(os.str());
+    }
+    std::sort(dims.begin(), dims.end());
+    std::ostringstream os;
+    os << "P" << P << (literal ? "L" : "F") << '|';
+    for (auto& d : dims) os << d << ',';
+    return os.str();
+}
+
+int find_variant1(const std::string& name) {
+    const auto& v = sweep1_variants();
+    for (size_t k = 0; k < v.size(); ++k)
+        if (name == v[k].name) return (int)k;
+    return -1;
+}
+int find_variant2(const std::string& name) {
+    const auto& v = sweep2_variants();
+    for (size_t k = 0; k < v.size(); ++k)
+        if (name == v[k].name) return (int)k;
+    return -1;
+}
+
+void load_tune_file(rsf_context* c) {
+    const char* path = std::getenv("RSF_TUNE_CACHE");
+    if (!path) return;
+    std::ifstream in(path);
+    std::string key, s1, s2;
+    int zc1, zc2;
+    while (in >> key >> s1 >> zc1 >> s2 >> zc2) c->tune[key] = TuneChoice{s1, s2, zc1, zc2};
+}
+
+void save_tune_file(rsf_context* c) {
+    const char* path = std::getenv("RSF_TUNE_CACHE");
+    if (!path) return;
+    std::ofstream out(path);
+    for (auto& kv : c->tune)
+        out << kv.first << ' ' << kv.second.s1 << ' ' << kv.second.zc1 << ' ' << kv.second.s2 << ' '
+            << kv.second.zc2 << '\n';
+}
+
+int max_dim_l(const std::vector<BlockDesc>& b) {
+    int m = 0;
+    for (auto& x : b) m = std::max(m, x.l);
+    return m;
+}
+
+// Chooses kernel variants and z-chunks for a problem: tuned if known, else defaults.
+void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const Derived& d, int literal,
+                   int* s1, int* s2, int* zc1, int* zc2) {
+    *s1 = default_sweep1(d.P, literal);
+    *s2 = default_sweep2(literal);
+    const int lmax = max_dim_l(blocks);
+    *zc1 = std::min(lmax, 128);
+    *zc2 = std::min(lmax, 64);
+    auto it = c->tune.find(tune_key(blocks, d.P, literal));
+    if (it != c->tune.end()) {
+        const int a = find_variant1(it->second.s1), b = find_variant2(it->second.s2);
+        if (a >= 0 && sweep1_variants()[a].P == d.P && sweep1_variants()[a].literal == literal) {
+            *s1 = a;
+            *zc1 = it->second.zc1;
+        }
+        if (b >= 0 && sweep2_variants()[b].literal == literal) {
+            *s2 = b;
+            *zc2 = it->second.zc2;
+        }
+    }
+}
+
+Runner* get_runner(rsf_context* c, const std::vector<BlockDesc>& blocks, int s1, int s2, int zc1, int zc2,
+                   int* status) {
+    std::ostringstream os;
+    os << blocks_key(blocks) << '|' << s1 << '|' << s2 << '|' << zc1 << '|' << zc2;
+    const std::string key = os.str();
+    for (auto it = c->runners.begin(); it != c->runners.end(); ++it) {
+        if ((*it)->key == key) {
+            c->runners.splice(c->runners.begin(), c->runners, it);
+            *status = RSF_OK;
+            return c->runners.front().get();
+        }
+    }
+    auto r = std::make_unique<Runner>();
+    r->key = key;
+    *status = plan_tiles(r->plan, blocks, s1, s2, zc1, zc2, c->num_cus);
+    if (*status != RSF_OK) return nullptr;
+    c->runners.push_front(std::move(r));
+    while (c->runners.size() > 96) c->runners.pop_back();
+    return c->runners.front().get();
+}
+
+int validate_blocks(const rsf_block* blocks, int nblocks, const Derived& d, std::vector<BlockDesc>* out,
+                    int64_t* total) {
+    if (!blocks || nblocks <= 0) return set_error(RSF_ERR_INVALID_ARGUMENT, "no blocks");
+    out->clear();
+    *total = 0;
+    int64_t end = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        const rsf_block& x = blocks[b];
+        if (x.w < d.P || x.h < d.P || x.l < d.P)
+            return set_error(RSF_ERR_INVALID_ARGUMENT,
+                             "block %d dims (%lld,%lld,%lld) are smaller than the %d-tap blur", b,
+                             (long long)x.w, (long long)x.h, (long long)x.l, d.P);
+        if (x.w > (1 << 30) || x.h > (1 << 30) || x.l > (1 << 30) || x.offset < 0)
+            return set_error(RSF_ERR_INVALID_ARGUMENT, "block %d out of range", b);
+        out->push_back(BlockDesc{x.offset, (int)x.w, (int)x.h, (int)x.l, 0});
+        *total += x.w * x.h * x.l;
+        end = std::max(end, x.offset + x.w * x.h * x.l);
+    }
+    // blocks must not overlap
+    std::vector<std::pair<int64_t, int64_t>> iv;
+    for (auto& x : *out) iv.push_back({x.off, x.off + (int64_t)x.w * x.h * x.l});
+    std::sort(iv.begin(), iv.end());
+    for (size_t k = 1; k < iv.size(); ++k)
+        if (iv[k].first < iv[k - 1].second) return set_error(RSF_ERR_INVALID_ARGUMENT, "blocks overlap");
+    *total = std::max(*total, end);  // buffer extent in voxels
+    return RSF_OK;
+}
+
+struct Prepared {
+    Derived d;
+    std::vector<BlockDesc> blocks;
+    int64_t extent = 0;  // voxels spanned by the packed buffers
+    int literal = 0, sw = 0;
+    Runner* runner = nullptr;
+    bool fused = true;
+};
+
+int prepare(rsf_context* c, const rsf_block* blocks, int nblocks, const rsf_params* p, Prepared* pr) {
+    RSF_TRY(derive(p, &pr->d));
+    RSF_TRY(validate_blocks(blocks, nblocks, pr->d, &pr->blocks, &pr->extent));
+    pr->literal = p->mode == RSF_MODE_LITERAL;
+    pr->sw = (int)p->sigma2;
+    int s1, s2, zc1, zc2;
+    pick_variants(c, pr->blocks, pr->d, pr->literal, &s1, &s2, &zc1, &zc2);
+    pr->fused = s1 >= 0 && s2 >= 0;
+    if (!pr->fused) return RSF_OK;
+    int st = RSF_OK;
+    pr->runner = get_runner(c, pr->blocks, s1, s2, zc1, zc2, &st);
+    return st;
+}
+
+int ensure_workspace(rsf_context* c, const Prepared& pr) {
+    const size_t n = (size_t)pr.extent * sizeof(float);
+    RSF_TRY(ensure_ws(c, c->phi2, n));
+    RSF_TRY(ensure_ws(c, c->tr, n));
+    RSF_TRY(ensure_ws(c, c->F, n));
+    if (!pr.literal) RSF_TRY(ensure_ws(c, c->blurI, n));
+    if (pr.sw > 0) {
+        RSF_TRY(ensure_ws(c, c->fo, n));
+        RSF_TRY(ensure_ws(c, c->fi, n));
+        RSF_TRY(ensure_ws(c, c->Eo, n));
+        RSF_TRY(ensure_ws(c, c->Ei, n));
+    }
+    if (!pr.literal || pr.sw > 0) RSF_TRY(ensure_ws(c, c->imgf, n));
+    if (!pr.literal) {
+        size_t tmax = 0;
+        for (auto& b : pr.blocks) tmax = std::max(tmax, (size_t)(b.w - pr.d.P + 1) * b.h * b.l);
+        RSF_TRY(ensure_ws(c, c->tmp1, tmax * sizeof(float)));
+        RSF_TRY(ensure_ws(c, c->tmp2, tmax * sizeof(float)));
+    }
+    return RSF_OK;
+}
+
+// Once per run: image as float and, in fast mode, the padded blur of the image.
+int prepare_image(rsf_context* c, const Prepared& pr, const void* image, int u8) {
+    if (pr.literal && pr.sw == 0) return RSF_OK;
+    RSF_TRY(ops::to_float(c->imgf.as<float>(), image, u8, pr.extent, c->stream));
+    if (pr.literal) return RSF_OK;
+    for (auto& b : pr.blocks)
+        RSF_TRY(ops::blur_pad(c->blurI.as<float>() + b.off, c->imgf.as<float>() + b.off, c->tmp1.as<float>(),
+                              c->tmp2.as<float>(), b.w, b.h, b.l, pr.d.taps, pr.d.P, pr.d.m, c->stream));
+    return RSF_OK;
+}
+
+// One iteration phi_in -> phi_out over the prepared blocks (tr must hold the
+// transform of phi_in; it is overwritten with the transform of phi_out).
+int enqueue_step(rsf_context* c, const Prepared& pr, const void* image, int u8, const float* phi_in,
+                 float* phi_out, const rsf_params* p) {
+    const TilePlan& tp = pr.runner->plan;
+    SweepArgs a{};
+    a.tr = c->tr.as<float>();
+    a.img = image;
+    a.img_u8 = u8;
+    a.blurI = c->blurI.as<float>();
+    if (pr.sw == 0) {
+        a.out0 = c->F.as<float>();
+        a.out1 = nullptr;
+        a.out_fits = 0;
+        RSF_TRY(launch_sweep1(tp, a, pr.d, p, c->stream));
+    } else {
+        a.out0 = c->fo.as<float>();
+        a.out1 = c->fi.as<float>();
+        a.out_fits = 1;
+        RSF_TRY(launch_sweep1(tp, a, pr.d, p, c->stream));
+        for (auto& b : pr.blocks)
+            RSF_TRY(ops::energies(c->Eo.as<float>() + b.off, c->Ei.as<float>() + b.off, c->fo.as<float>() + b.off,
+                                  c->fi.as<float>() + b.off, c->imgf.as<float>() + b.off, b.w, b.h, b.l, pr.sw,
+                                  c->stream));
+        RSF_TRY(ops::force(c->F.as<float>(), c->Eo.as<float>(), c->Ei.as<float>(), p->lambda1, p->lambda2,
+                           pr.extent, c->stream));
+    }
+    a.phi = phi_in;
+    a.F = c->F.as<float>();
+    a.phi_out = phi_out;
+    a.tr_out = c->tr.as<float>();
+    return launch_sweep2(tp, a, p, c->stream);
+}
+
+// Captures n iterations ping-ponging between A (input) and B; when n is odd
+// and copy_back is set, a device copy brings the result back into A.
+int capture(rsf_context* c, const Prepared& pr, const void* image, int u8, float* A0, float* B0,
+            const rsf_params* p, int n, bool copy_back, hipGraphExec_t* out) {
+    hipGraph_t g = nullptr;
+    RSF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int st = RSF_OK;
+    float* A = A0;
+    float* B = B0;
+    for (int t = 0; t < n && st == RSF_OK; ++t) {
+        st = enqueue_step(c, pr, image, u8, A, B, p);
+        std::swap(A, B);
+    }
+    if (st == RSF_OK && copy_back && A != A0)
+        st = (hipMemcpyAsync(A0, A, (size_t)pr.extent * sizeof(float), hipMemcpyDeviceToDevice, c->stream) ==
+              hipSuccess) ? RSF_OK : set_error(RSF_ERR_DEVICE, "hipMemcpyAsync in capture failed");
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st != RSF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    return RSF_OK;
+}
+
+int run_prepared(rsf_context* c, const Prepared& pr, const void* image, int u8, float* phi, const rsf_params* p,
+                 float* iter_ms) {
+    RSF_TRY(ensure_workspace(c, pr));
+    RSF_TRY(prepare_image(c, pr, image, u8));
+    RSF_TRY(launch_transform(c->tr.as<float>(), phi, pr.extent, pr.literal, p->epsilon, c->stream));
+    const int T = p->iterations;
+    if (T == 0) return RSF_OK;
+    Runner* r = pr.runner;
+    std::ostringstream gk;
+    gk << image << '|' << u8 << '|' << (const void*)phi << '|' << p->sigma << '|' << p->lambda1 << '|'
+       << p->lambda2 << '|' << p->nu << '|' << p->mu << '|' << p->dt << '|' << p->epsilon << '|' << p->sigma2
+       << '|' << T << '|' << (iter_ms ? 1 : 0);
+    if (r->graph_key != gk.str() || r->ws_epoch != c->ws_epoch) {
+        r->graphs.reset();
+        r->graph_key.clear();
+        float* B = c->phi2.as<float>();
+        if (iter_ms) {
+            // one graph per direction, launched alternately, events in between
+            RSF_TRY(capture(c, pr, image, u8, phi, B, p, 1, false, &r->graphs.main));
+            if (T > 1) RSF_TRY(capture(c, pr, image, u8, B, phi, p, 1, false, &r->graphs.rem));
+            r->graphs.n_main = 1;
+            r->graphs.n_launch = T;
+        } else {
+            const int chunk = std::min(T, 64) & ~1;  // even: each launch ends in phi
+            if (chunk > 0) RSF_TRY(capture(c, pr, image, u8, phi, B, p, chunk, true, &r->graphs.main));
+            r->graphs.n_main = chunk;
+            r->graphs.n_launch = chunk > 0 ? T / chunk : 0;
+            r->graphs.n_rem = chunk > 0 ? T % chunk : T;
+            if (r->graphs.n_rem)
+                RSF_TRY(capture(c, pr, image, u8, phi, B, p, r->graphs.n_rem, true, &r->graphs.rem));
+        }
+        r->graph_key = gk.str();
+        r->ws_epoch = c->ws_epoch;
+    }
+    if (iter_ms) {
+        if ((int)c->iter_events.size() < T + 1) {
+            for (int k = (int)c->iter_events.size(); k < T + 1; ++k) {
+                hipEvent_t e;
+                RSF_HIP(hipEventCreate(&e));
+                c->iter_events.push_back(e);
+            }
+        }
+        RSF_HIP(hipEventRecord(c->iter_events[0], c->stream));
+        for (int t = 0; t < T; ++t) {
+            RSF_HIP(hipGraphLaunch((t & 1) ? r->graphs.rem : r->graphs.main, c->stream));
+            RSF_HIP(hipEventRecord(c->iter_events[t + 1], c->stream));
+        }
+        if (T & 1)
+            RSF_HIP(hipMemcpyAsync(phi, c->phi2.p, (size_t)pr.extent * sizeof(float), hipMemcpyDeviceToDevice,
+                                   c->stream));
+        RSF_HIP(hipStreamSynchronize(c->stream));
+        for (int t = 0; t < T; ++t) RSF_HIP(hipEventElapsedTime(&iter_ms[t], c->iter_events[t], c->iter_events[t + 1]));
+        RSF_HIP(hipEventElapsedTime(&c->last_ms, c->iter_events[0], c->iter_events[T]));
+        return RSF_OK;
+    }
+    RSF_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int k = 0; k < r->graphs.n_launch; ++k) RSF_HIP(hipGraphLaunch(r->graphs.main, c->stream));
+    if (r->graphs.rem) RSF_HIP(hipGraphLaunch(r->graphs.rem, c->stream));
+    RSF_HIP(hipEventRecord(c->ev1, c->stream));
+    RSF_HIP(hipEventSynchronize(c->ev1));
+    RSF_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    return RSF_OK;
+}
+
+int check_ctx(rsf_context* c) {
+    if (!c) return set_error(RSF_ERR_INVALID_ARGUMENT, "context is NULL");
+    RSF_HIP(hipSetDevice(c->device));
+    return RSF_OK;
+}
+
+int check_dtype(int dtype) {
+    if (dtype != RSF_DTYPE_U8 && dtype != RSF_DTYPE_F32)
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "image dtype must be RSF_DTYPE_U8 or RSF_DTYPE_F32");
+    return RSF_OK;
+}
+
+int unfused_step(rsf_context* c, const float* I, const float* phi, float* out, int w, int h, int l,
+                 const rsf_params* p, const Derived& d);
+
+}  // namespace
+
+extern "C" {
+
+int rsf_abi_version(void) { return RSF_ABI_VERSION; }
+
+const char* rsf_last_error(void) { return g_last_error.c_str(); }
+
+const char* rsf_status_string(int s) {
+    switch (s) {
+        case RSF_OK: return "ok";
+        case RSF_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case RSF_ERR_UNSUPPORTED: return "unsupported";
+        case RSF_ERR_OUT_OF_MEMORY: return "out of memory";
+        case RSF_ERR_DEVICE: return "device error";
+        case RSF_ERR_IO: return "i/o error";
+        case RSF_ERR_NONFINITE: return "non-finite values";
+        default: return "unknown status";
+    }
+}
+
+void rsf_params_default(rsf_params* p) {
+    if (!p) return;
+    p->sigma = 5.0f;      // main.cpp:91
+    p->iterations = 30;   // :92
+    p->dt = 0.1f;         // :93
+    p->mu = 1.0f;         // :94
+    p->lambda1 = 0.1f;    // :95 (e)
+    p->lambda2 = 0.1f;
+    p->nu = 0.0001f;      // :96
+    p->sigma2 = 0.0f;     // :97
+    p->epsilon = 0.2f;    // kernel.cu:258
+    p->mode = RSF_MODE_FAST;
+}
+
+int rsf_taps(float sigma, float* taps, int capacity, int* ntaps, int* pad) {
+    if (!taps || !ntaps || !pad) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL output");
+    return taps32(sigma, taps, capacity, ntaps, pad);
+}
+
+int rsf_context_create(int device, void* hip_stream, rsf_context** out) {
+    if (!out) return set_error(RSF_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    RSF_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_error(RSF_ERR_INVALID_ARGUMENT, "device %d of %d", device, n);
+    RSF_HIP(hipSetDevice(device));
+    auto c = std::make_unique<rsf_context>();
+    c->device = device;
+    if (hip_stream) {
+        c->stream = (hipStream_t)hip_stream;
+    } else {
+        RSF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    hipDeviceProp_t prop;
+    RSF_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    RSF_HIP(hipEventCreate(&c->ev0));
+    RSF_HIP(hipEventCreate(&c->ev1));
+    load_tune_file(c.get());
+    *out = c.release();
+    return RSF_OK;
+}
+
+int rsf_context_destroy(rsf_context* c) {
+    if (!c) return RSF_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->runners.clear();
+    for (auto e : c->iter_events) (void)hipEventDestroy(e);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RSF_OK;
+}
+
+int rsf_context_set_stream(rsf_context* c, void* hip_stream) {
+    RSF_TRY(check_ctx(c));
+    hipStream_t s = hip_stream ? (hipStream_t)hip_stream : nullptr;
+    if (!s) return set_error(RSF_ERR_INVALID_ARGUMENT, "stream is NULL");
+    if (s == c->stream) return RSF_OK;
+    RSF_HIP(hipStreamSynchronize(c->stream));
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    c->stream = s;
+    c->own_stream = false;
+    return RSF_OK;
+}
+
+int rsf_workspace_size(const rsf_params* p, int64_t w, int64_t h, int64_t l, int64_t* bytes) {
+    if (!bytes) return set_error(RSF_ERR_INVALID_ARGUMENT, "bytes is NULL");
+    Derived d;
+    RSF_TRY(derive(p, &d));
+    const int64_t n = w * h * l;
+    int64_t vols = 3;  // phi ping-pong, transform, force
+    if (p->mode == RSF_MODE_FAST) vols += 2;           // blur(I), image as float
+    if ((int)p->sigma2 > 0) vols += 4 + (p->mode == RSF_MODE_FAST ? 0 : 1);
+    int64_t tmp = (p->mode == RSF_MODE_FAST) ? 2 * (w - d.P + 1) * h * l : 0;
+    *bytes = (vols * n + tmp) * (int64_t)sizeof(float);
+    return RSF_OK;
+}
+
+int rsf_last_loop_ms(rsf_context* c, float* ms) {
+    if (!c || !ms) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL argument");
+    *ms = c->last_ms;
+    return RSF_OK;
+}
+
+int rsf_step_device(rsf_context* c, const void* image, int image_dtype, const float* phi_in, float* phi_out,
+                    int64_t w, int64_t h, int64_t l, const rsf_params* p) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    if (!image || !phi_in || !phi_out) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (phi_in == phi_out) return set_error(RSF_ERR_INVALID_ARGUMENT, "phi_in and phi_out must not alias");
+    rsf_block b{0, w, h, l};
+    return rsf_step_blocks_device(c, &b, 1, image, image_dtype, phi_in, phi_out, p);
+}
+
+int rsf_step_blocks_device(rsf_context* c, const rsf_block* blocks, int nblocks, const void* image,
+                           int image_dtype, const float* phi_in, float* phi_out, const rsf_params* p) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    if (!image || !phi_in || !phi_out) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    Prepared pr;
+    RSF_TRY(prepare(c, blocks, nblocks, p, &pr));
+    const int u8 = image_dtype == RSF_DTYPE_U8;
+    if (!pr.fused) {
+        if (nblocks != 1) return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel", pr.d.P);
+        const BlockDesc& b = pr.blocks[0];
+        RSF_TRY(ensure_ws(c, c->imgf, (size_t)pr.extent * sizeof(float)));
+        RSF_TRY(ops::to_float(c->imgf.as<float>(), image, u8, pr.extent, c->stream));
+        RSF_TRY(unfused_step(c, c->imgf.as<float>() + b.off, phi_in + b.off, phi_out + b.off, b.w, b.h, b.l, p, pr.d));
+        RSF_HIP(hipStreamSynchronize(c->stream));
+        return RSF_OK;
+    }
+    RSF_TRY(ensure_workspace(c, pr));
+    RSF_TRY(prepare_image(c, pr, image, u8));
+    RSF_TRY(launch_transform(c->tr.as<float>(), phi_in, pr.extent, pr.literal, p->epsilon, c->stream));
+    RSF_TRY(enqueue_step(c, pr, image, u8, phi_in, phi_out, p));
+    RSF_HIP(hipStreamSynchronize(c->stream));
+    return RSF_OK;
+}
+
+int rsf_run_device(rsf_context* c, const void* image, int image_dtype, float* phi, int64_t w, int64_t h,
+                   int64_t l, const rsf_params* p, float* iter_ms) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    if (!image || !phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    rsf_block b{0, w, h, l};
+    Prepared pr;
+    RSF_TRY(prepare(c, &b, 1, p, &pr));
+    const int u8 = image_dtype == RSF_DTYPE_U8;
+    if (!pr.fused) {
+        // unfused pipeline for tap counts without a fused specialisation
+        RSF_TRY(ensure_ws(c, c->imgf, (size_t)pr.extent * sizeof(float)));
+        RSF_TRY(ensure_ws(c, c->phi2, (size_t)pr.extent * sizeof(float)));
+        RSF_TRY(ops::to_float(c->imgf.as<float>(), image, u8, pr.extent, c->stream));
+        float* A = phi;
+        float* B = c->phi2.as<float>();
+        RSF_HIP(hipEventRecord(c->ev0, c->stream));
+        for (int t = 0; t < p->iterations; ++t) {
+            if (iter_ms) RSF_HIP(hipEventRecord(c->ev0, c->stream));
+            RSF_TRY(unfused_step(c, c->imgf.as<float>(), A, B, (int)w, (int)h, (int)l, p, pr.d));
+            std::swap(A, B);
+            if (iter_ms) {
+                RSF_HIP(hipEventRecord(c->ev1, c->stream));
+                RSF_HIP(hipEventSynchronize(c->ev1));
+                RSF_HIP(hipEventElapsedTime(&iter_ms[t], c->ev0, c->ev1));
+            }
+        }
+        if (A != phi) RSF_HIP(hipMemcpyAsync(phi, A, (size_t)pr.extent * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        RSF_HIP(hipStreamSynchronize(c->stream));
+        return RSF_OK;
+    }
+    return run_prepared(c, pr, image, u8, phi, p, iter_ms);
+}
+
+int rsf_run_host(rsf_context* c, const void* image, int image_dtype, float* phi, int64_t w, int64_t h, int64_t l,
+                 const rsf_params* p, float* iter_ms) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    if (!image || !phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (w <= 0 || h <= 0 || l <= 0) return set_error(RSF_ERR_INVALID_ARGUMENT, "dims must be positive");
+    const int64_t n = w * h * l;
+    const size_t ib = (size_t)n * (image_dtype == RSF_DTYPE_U8 ? 1 : 4);
+    RSF_TRY(c->h_img.ensure(ib));
+    RSF_TRY(c->h_phi.ensure((size_t)n * sizeof(float)));
+    RSF_HIP(hipMemcpyAsync(c->h_img.p, image, ib, hipMemcpyHostToDevice, c->stream));
+    RSF_HIP(hipMemcpyAsync(c->h_phi.p, phi, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    RSF_TRY(rsf_run_device(c, c->h_img.p, image_dtype, c->h_phi.as<float>(), w, h, l, p, iter_ms));
+    RSF_HIP(hipMemcpyAsync(phi, c->h_phi.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    RSF_HIP(hipStreamSynchronize(c->stream));
+    return RSF_OK;
+}
+
+int rsf_run_batched_device(rsf_context* c, const rsf_block* blocks, int nblocks, const void* image, int image_dtype,
+                           float* phi, const rsf_params* p) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    if (!image || !phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    Prepared pr;
+    RSF_TRY(prepare(c, blocks, nblocks, p, &pr));
+    if (!pr.fused) return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel for batched runs", pr.d.P);
+    return run_prepared(c, pr, image, image_dtype == RSF_DTYPE_U8, phi, p, nullptr);
+}
+
+int rsf_profile_sweeps(rsf_context* c, const rsf_block* blocks, int nblocks, const void* image, int image_dtype,
+                       float* phi, const rsf_params* p, float* ms_sweep1, float* ms_sweep2) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    if (!image || !phi || !ms_sweep1 || !ms_sweep2) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL argument");
+    Prepared pr;
+    RSF_TRY(prepare(c, blocks, nblocks, p, &pr));
+    if (!pr.fused || pr.sw != 0) return set_error(RSF_ERR_UNSUPPORTED, "profiling covers the fused sigma2 = 0 path");
+    const int u8 = image_dtype == RSF_DTYPE_U8;
+    RSF_TRY(ensure_workspace(c, pr));
+    RSF_TRY(prepare_image(c, pr, image, u8));
+    RSF_TRY(launch_transform(c->tr.as<float>(), phi, pr.extent, pr.literal, p->epsilon, c->stream));
+    const int T = std::max(1, (int)p->iterations);
+    std::vector<hipEvent_t> ev(3 * T);
+    for (auto& e : ev) RSF_HIP(hipEventCreate(&e));
+    const TilePlan& tp = pr.runner->plan;
+    float* A = phi;
+    float* B = c->phi2.as<float>();
+    int st = RSF_OK;
+    for (int t = 0; t < T && st == RSF_OK; ++t) {
+        SweepArgs a{};
+        a.tr = c->tr.as<float>();
+        a.img = image;
+        a.img_u8 = u8;
+        a.blurI = c->blurI.as<float>();
+        a.out0 = c->F.as<float>();
+        a.phi = A;
+        a.F = c->F.as<float>();
+        a.phi_out = B;
+        a.tr_out = c->tr.as<float>();
+        (void)hipEventRecord(ev[3 * t], c->stream);
+        st = launch_sweep1(tp, a, pr.d, p, c->stream);
+        (void)hipEventRecord(ev[3 * t + 1], c->stream);
+        if (st == RSF_OK) st = launch_sweep2(tp, a, p, c->stream);
+        (void)hipEventRecord(ev[3 * t + 2], c->stream);
+        std::swap(A, B);
+    }
+    if (st == RSF_OK && A != phi)
+        st = hipMemcpyAsync(phi, A, (size_t)pr.extent * sizeof(float), hipMemcpyDeviceToDevice, c->stream) == hipSuccess
+                 ? RSF_OK : set_error(RSF_ERR_DEVICE, "copy back failed");
+    (void)hipStreamSynchronize(c->stream);
+    double s1 = 0, s2 = 0;
+    for (int t = 0; t < T && st == RSF_OK; ++t) {
+        float a = 0, b = 0;
+        (void)hipEventElapsedTime(&a, ev[3 * t], ev[3 * t + 1]);
+        (void)hipEventElapsedTime(&b, ev[3 * t + 1], ev[3 * t + 2]);
+        s1 += a;
+        s2 += b;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    RSF_TRY(st);
+    *ms_sweep1 = (float)(s1 / T);
+    *ms_sweep2 = (float)(s2 / T);
+    return RSF_OK;
+}
+
+int rsf_run_blocks_sequential_device(rsf_context* c, const rsf_block* blocks, int nblocks, const void* image,
+                                     int image_dtype, float* phi, const rsf_params* p) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    if (!blocks || nblocks <= 0) return set_error(RSF_ERR_INVALID_ARGUMENT, "no blocks");
+    const size_t esz = image_dtype == RSF_DTYPE_U8 ? 1 : 4;
+    float total = 0.f;
+    for (int b = 0; b < nblocks; ++b) {
+        const rsf_block& x = blocks[b];
+        RSF_TRY(rsf_run_device(c, (const char*)image + x.offset * esz, image_dtype, phi + x.offset, x.w, x.h, x.l, p,
+                               nullptr));
+        total += c->last_ms;
+    }
+    c->last_ms = total;
+    return RSF_OK;
+}
+
+int rsf_autotune(rsf_context* c, const rsf_block* blocks, int nblocks, int image_dtype, const rsf_params* p,
+                 char* choice, int choice_len) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    Derived d;
+    RSF_TRY(derive(p, &d));
+    std::vector<BlockDesc> bl;
+    int64_t extent = 0;
+    RSF_TRY(validate_blocks(blocks, nblocks, d, &bl, &extent));
+    const int literal = p->mode == RSF_MODE_LITERAL;
+    const size_t n = (size_t)extent;
+    // synthetic operands: a ramp level set and a constant image
+    DevBuf img, phi, out, tr, F, fo, fi, bI;
+    RSF_TRY(img.ensure(n * (image_dtype == RSF_DTYPE_U8 ? 1 : 4)));
+    RSF_TRY(phi.ensure(n * 4));
+    RSF_TRY(out.ensure(n * 4));
+    RSF_TRY(tr.ensure(n * 4));
+    RSF_TRY(F.ensure(n * 4));
+    RSF_TRY(bI.ensure(n * 4));
+    RSF_HIP(hipMemsetAsync(img.p, 0x40, img.bytes, c->stream));
+    RSF_HIP(hipMemsetAsync(phi.p, 0, phi.bytes, c->stream));
+    RSF_HIP(hipMemsetAsync(tr.p, 0, tr.bytes, c->stream));
+    RSF_HIP(hipMemsetAsync(bI.p, 0, bI.bytes, c->stream));
+    const int lmax = max_dim_l(bl);
+    auto time_it = [&](auto launch) -> float {
+        for (int k = 0; k < 2; ++k)
+            if (launch() != RSF_OK) return 1e30f;
+        if (hipEventRecord(c->ev0, c->stream) != hipSuccess) return 1e30f;
+        for (int k = 0; k < 5; ++k)
+            if (launch() != RSF_OK) return 1e30f;
+        (void)hipEventRecord(c->ev1, c->stream);
+        (void)hipEventSynchronize(c->ev1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        return ms / 5;
+    };
+    SweepArgs a{};
+    a.tr = tr.as<float>();
+    a.img = img.p;
+    a.img_u8 = image_dtype == RSF_DTYPE_U8;
+    a.blurI = bI.as<float>();
+    a.out0 = F.as<float>();
+    a.phi = phi.as<float>();
+    a.F = F.as<float>();
+    a.phi_out = out.as<float>();
+    a.tr_out = tr.as<float>();
+    const auto& V1 = sweep1_variants();
+    const auto& V2 = sweep2_variants();
+    float best1 = 1e30f, best2 = 1e30f;
+    TuneChoice tc;
+    std::vector<int> zcs = {32, 64, 128, 256, lmax};
+    for (size_t k = 0; k < V1.size(); ++k) {
+        if (V1[k].P != d.P || V1[k].literal != literal) continue;
+        for (int zc : zcs) {
+            if (zc > lmax) continue;
+            TilePlan tp;
+            if (plan_tiles(tp, bl, (int)k, default_sweep2(literal), zc, std::min(lmax, 64), c->num_cus) != RSF_OK)
+                continue;
+            const float ms = time_it([&] { return launch_sweep1(tp, a, d, p, c->stream); });
+            if (ms < best1) { best1 = ms; tc.s1 = V1[k].name; tc.zc1 = zc; }
+        }
+    }
+    std::vector<int> zc2s = {8, 16, 32, 64, lmax};
+    for (size_t k = 0; k < V2.size(); ++k) {
+        if (V2[k].literal != literal) continue;
+        for (int zc : zc2s) {
+            if (zc > lmax) continue;
+            const int s1 = default_sweep1(d.P, literal);
+            if (s1 < 0) break;
+            TilePlan tp;
+            if (plan_tiles(tp, bl, s1, (int)k, std::min(lmax, 128), zc, c->num_cus) != RSF_OK) continue;
+            const float ms = time_it([&] { return launch_sweep2(tp, a, p, c->stream); });
+            if (ms < best2) { best2 = ms; tc.s2 = V2[k].name; tc.zc2 = zc; }
+        }
+    }
+    RSF_HIP(hipStreamSynchronize(c->stream));
+    if (tc.s1.empty() || tc.s2.empty()) return set_error(RSF_ERR_UNSUPPORTED, "no fused variant for P=%d", d.P);
+    c->tune[tune_key(bl, d.P, literal)] = tc;
+    save_tune_file(c);
+    if (choice && choice_len > 0) {
+        snprintf(choice, (size_t)choice_len, "sweep1=%s zc=%d (%.3f ms) sweep2=%s zc=%d (%.3f ms)", tc.s1.c_str(),
+                 tc.zc1, best1, tc.s2.c_str(), tc.zc2, best2);
+    }
+    return RSF_OK;
+}
+
+int rsf_step_unfused_device(rsf_context* c, const float* image, const float* phi_in, float* phi_out, int64_t w,
+                            int64_t h, int64_t l, const rsf_params* p) {
+    RSF_TRY(check_ctx(c));
+    if (!image || !phi_in || !phi_out) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    Derived d;
+    RSF_TRY(derive(p, &d));
+    if (w < d.P || h < d.P || l < d.P) return set_error(RSF_ERR_INVALID_ARGUMENT, "dims smaller than the blur");
+    RSF_TRY(unfused_step(c, image, phi_in, phi_out, (int)w, (int)h, (int)l, p, d));
+    RSF_HIP(hipStreamSynchronize(c->stream));
+    return RSF_OK;
+}
+
+int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks, const void* image, int image_dtype,
+                            const float* phi, const float* grad_out, float* grad_in, float* grad_params,
+                            const rsf_params* p) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    if (!image || !phi || !grad_out || !grad_in || !grad_params)
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    Prepared pr;
+    RSF_TRY(prepare(c, blocks, nblocks, p, &pr));
+    if (!pr.fused) return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel", pr.d.P);
+    RSF_TRY(adjoint_step(c, pr.runner->plan, image, image_dtype == RSF_DTYPE_U8, phi, grad_out, grad_in, grad_params,
+                         p, pr.d, c->stream));
+    RSF_HIP(hipStreamSynchronize(c->stream));
+    return RSF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// main.cpp:234-300 op by op, FP32-literal, on the device (kernels of rsf_ops.hip).
+int unfused_step(rsf_context* c, const float* I, const float* phi, float* out, int w, int h, int l,
+                 const rsf_params* p, const Derived& d) {
+    const int64_t n = (int64_t)w * h * l;
+    const size_t nb = (size_t)n * sizeof(float);
+    for (int k = 0; k < 20; ++k) RSF_TRY(ensure_ws(c, c->unf[k], nb));
+    float* U[20];
+    for (int k = 0; k < 20; ++k) U[k] = c->unf[k].as<float>();
+    float *H = U[0], *A = U[1], *Hc = U[2], *B = U[3], *gx = U[4], *gy = U[5], *gz = U[6], *M = U[7];
+    float *nx = U[8], *ny = U[9], *nz = U[10], *tx = U[11], *ty = U[12], *tz = U[13], *kap = U[14], *L = U[15];
+    float *p1 = U[16], *p2 = U[17], *fo = U[18], *fi = U[19];
+    hipStream_t s = c->stream;
+    const int P = d.P, m = d.m;
+    RSF_TRY(ops::heaviside(H, phi, n, p->epsilon, s));
+    RSF_TRY(ops::multiplication(A, I, H, n, s));
+    RSF_TRY(ops::inverse(Hc, H, n, s));
+    RSF_TRY(ops::multiplication(B, I, Hc, n, s));
+    RSF_TRY(ops::gradient(gx, gy, gz, phi, w, h, l, s));
+    RSF_TRY(ops::gradientmagnitude(M, gx, gy, gz, n, s));
+    RSF_TRY(ops::division_grad(nx, gx, M, n, s));
+    RSF_TRY(ops::division_grad(ny, gy, M, n, s));
+    RSF_TRY(ops::division_grad(nz, gz, M, n, s));
+    RSF_TRY(ops::gradient(tx, nullptr, nullptr, nx, w, h, l, s));
+    RSF_TRY(ops::gradient(nullptr, ty, nullptr, ny, w, h, l, s));
+    RSF_TRY(ops::gradient(nullptr, nullptr, tz, nz, w, h, l, s));
+    RSF_TRY(ops::plus(kap, tx, ty, tz, n, s));
+    RSF_TRY(ops::gradient(tx, nullptr, nullptr, gx, w, h, l, s));
+    RSF_TRY(ops::gradient(nullptr, ty, nullptr, gy, w, h, l, s));
+    RSF_TRY(ops::gradient(nullptr, nullptr, tz, gz, w, h, l, s));
+    RSF_TRY(ops::plus(L, tx, ty, tz, n, s));
+    // blurs: reuse gx/gy (no longer needed) as pass temporaries, M/nx as the blurred outputs
+    RSF_TRY(ops::blur_pad(p1, A, gx, gy, w, h, l, d.taps, P, m, s));
+    RSF_TRY(ops::blur_pad(p2, H, gx, gy, w, h, l, d.taps, P, m, s));
+    RSF_TRY(ops::division(fo, p1, p2, n, s));
+    RSF_TRY(ops::blur_pad(p1, B, gx, gy, w, h, l, d.taps, P, m, s));
+    RSF_TRY(ops::blur_pad(p2, Hc, gx, gy, w, h, l, d.taps, P, m, s));
+    RSF_TRY(ops::division(fi, p1, p2, n, s));
+    float *Eo = A, *Ei = B, *dh = Hc;
+    RSF_TRY(ops::energies(Eo, Ei, fo, fi, I, w, h, l, (int)p->sigma2, s));
+    RSF_TRY(ops::deri_heaviside(dh, phi, n, p->epsilon, s));
+    return ops::final_phi(out, phi, dh, Eo, Ei, kap, L, p->lambda1, p->lambda2, p->dt, p->nu, p->mu, n, s);
+}
+
+}  // namespace

@@ -1,0 +1,116 @@
+// Device-side helpers shared by the RSF kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+namespace rsf {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kMaxTaps = 63;
+
+// Taps travel as a by-value kernel argument, so they live in SGPRs.
+struct Taps {
+    float g[kMaxTaps];
+};
+
+// One sub-volume of a (possibly batched) run: first voxel and dims.
+struct BlockDesc {
+    int64_t off;
+    int w, h, l, pad_;
+};
+
+// A unit of work of the tiled sweeps: block index (< 0 = empty slot) and the
+// tile's first voxel inside the block.
+struct Tile {
+    int b, x0, y0, z0;
+};
+
+// ---------------------------------------------------------------- point ops
+// Heaviside of CUDA_RSF/kernel.cu:258-259: float atan, double 2/pi mul-add.
+// Contraction is off inside the literal helpers so that each operation is
+// rounded on its own, as in the oracle.
+__device__ __forceinline__ float heav_literal(float phi, float eps) {
+#pragma clang fp contract(off)
+    return (float)(0.5 * (1.0 + (2.0 / kPi) * (double)atanf(phi / eps)));
+}
+// Dirac of kernel.cu:275-276: float quotient scaled by the double 1/pi.
+__device__ __forceinline__ float dirac_literal(float phi, float eps) {
+#pragma clang fp contract(off)
+    return (float)((1.0 / kPi) * (double)(eps / ((eps * eps) + (phi * phi))));
+}
+// Centred Heaviside G = H - 1/2 = atan(phi/eps)/pi (FP32).
+__device__ __forceinline__ float heav_centred(float phi, float inv_eps) {
+    return atanf(phi * inv_eps) * (float)(1.0 / kPi);
+}
+__device__ __forceinline__ float dirac_fast(float phi, float eps) {
+    return (float)(1.0 / kPi) * (eps * __builtin_amdgcn_rcpf((eps * eps) + (phi * phi)));
+}
+
+// ------------------------------------------------- difference operator rules
+// D f(q) = (f(R) - f(L)) / 2 with the reference edge rules (kernel.cu:57-125):
+//   x: R = q+1 (q at x = w-1), L = q-1 in linear order (wraps to the previous
+//      row at x = 0; element 0 at linear index 0);
+//   y, z: interior central, one-sided halves at both ends.
+struct Grad {
+    float x, y, z;
+};
+
+__device__ __forceinline__ Grad grad_at(const float* __restrict__ f, int qx, int qy, int qz,
+                                        int w, int h, int l) {
+    const int64_t plane = (int64_t)w * h;
+    const int64_t qi = (int64_t)qz * plane + (int64_t)qy * w + qx;
+    const int64_t xr = (qx == w - 1) ? qi : qi + 1;
+    const int64_t xl = (qi == 0) ? 0 : qi - 1;
+    int64_t yr, yl, zr, zl;
+    if (qy == 0) { yr = qi + w; yl = qi; }
+    else if (qy == h - 1) { yr = qi; yl = qi - w; }
+    else { yr = qi + w; yl = qi - w; }
+    if (qz == 0) { zr = qi + plane; zl = qi; }
+    else if (qz == l - 1) { zr = qi; zl = qi - plane; }
+    else { zr = qi + plane; zl = qi - plane; }
+    Grad g;
+    g.x = (f[xr] - f[xl]) * 0.5f;
+    g.y = (f[yr] - f[yl]) * 0.5f;
+    g.z = (f[zr] - f[zl]) * 0.5f;
+    return g;
+}
+
+// Coordinates of the D_x left operand of voxel (x,y,z): the previous linear
+// element.
+__device__ __forceinline__ void left_x(int x, int y, int z, int w, int h, int& qx, int& qy,
+                                       int& qz) {
+    if (x > 0) { qx = x - 1; qy = y; qz = z; }
+    else if (y > 0) { qx = w - 1; qy = y - 1; qz = z; }
+    else if (z > 0) { qx = w - 1; qy = h - 1; qz = z - 1; }
+    else { qx = 0; qy = 0; qz = 0; }
+}
+
+// ------------------------------------------------ register ring (z pushes)
+// Input plane t contributes tap k to the output whose first plane is t-k;
+// slot s of the ring holds the output whose first plane is = s (mod P).
+// Each slot starts from v*g[0] and accumulates k = 1..P-1 in order, the
+// summation order of Convolution__on_Z (kernel.cu:1044-1045).
+template <int P, int NC, int S>
+__device__ __forceinline__ void zpush(float (&acc)[NC][P], const float (&v)[NC], const Taps& tp,
+                                      float (&done)[NC]) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+        acc[n][S] = v[n] * tp.g[0];
+#pragma unroll
+        for (int k = 1; k < P; ++k)
+            acc[n][(S - k + P) % P] = fmaf(v[n], tp.g[k], acc[n][(S - k + P) % P]);
+        done[n] = acc[n][(S + 1) % P];
+    }
+}
+
+template <int P, int NC, int... S>
+__device__ __forceinline__ void zdispatch(int s, float (&acc)[NC][P], const float (&v)[NC],
+                                          const Taps& tp, float (&done)[NC],
+                                          std::integer_sequence<int, S...>) {
+    ((s == S ? zpush<P, NC, S>(acc, v, tp, done) : void()), ...);
+}
+
+}  // namespace rsf

@@ -1,0 +1,141 @@
+// Host-side internals shared by the librsf translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "rsf.h"
+#include "rsf_device.hpp"
+
+namespace rsf {
+
+// ------------------------------------------------------------ error plumbing
+int set_error(int status, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define RSF_HIP(expr)                                                         \
+    do {                                                                      \
+        hipError_t rsf_e_ = (expr);                                           \
+        if (rsf_e_ != hipSuccess) return ::rsf::hip_fail(rsf_e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define RSF_TRY(expr)                   \
+    do {                                \
+        int rsf_s_ = (expr);            \
+        if (rsf_s_ != RSF_OK) return rsf_s_; \
+    } while (0)
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+    ~DevBuf() { release(); }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Derived quantities of a parameter set.
+struct Derived {
+    int P = 0, m = 0;
+    Taps taps{};
+    float halfS3 = 0.f;   // (sum g)^3 / 2: blur of the constant 1/2
+};
+int derive(const rsf_params* p, Derived* d);
+int check_params(const rsf_params* p);
+
+// ------------------------------------------------------ per-op kernel launches
+// Literal mirrors of the reference kernels (rsf_ops.hip).
+namespace ops {
+int heaviside(float* out, const float* in, int64_t n, float eps, hipStream_t s);
+int deri_heaviside(float* out, const float* in, int64_t n, float eps, hipStream_t s);
+int multiplication(float* out, const float* a, const float* b, int64_t n, hipStream_t s);
+int inverse(float* out, const float* a, int64_t n, hipStream_t s);
+int plus(float* out, const float* a, const float* b, const float* c, int64_t n, hipStream_t s);
+int division(float* out, const float* a, const float* b, int64_t n, hipStream_t s);
+int division_grad(float* out, const float* g, const float* M, int64_t n, hipStream_t s);
+int gradientmagnitude(float* M, const float* gx, const float* gy, const float* gz, int64_t n,
+                      hipStream_t s);
+// any of gx/gy/gz may be null
+int gradient(float* gx, float* gy, float* gz, const float* f, int w, int h, int l, hipStream_t s);
+// valid correlation along one axis (0=x,1=y,2=z) of a (w,h,l) input
+int conv_axis(float* out, const float* in, int w, int h, int l, int axis, const Taps& t, int P,
+              hipStream_t s);
+int pad_border(float* out, const float* in, int vw, int vh, int vl, int pad, hipStream_t s);
+int energies(float* Eo, float* Ei, const float* fo, const float* fi, const float* I, int w, int h,
+             int l, int sw, hipStream_t s);
+int final_phi(float* out, const float* phi, const float* dh, const float* Eo, const float* Ei,
+              const float* kap, const float* L, float l1, float l2, float dt, float nu, float mu,
+              int64_t n, hipStream_t s);
+// F = lambda*(Eo-Ei) or lambda1*Eo - lambda2*Ei
+int force(float* F, const float* Eo, const float* Ei, float l1, float l2, int64_t n, hipStream_t s);
+// image (u8 or f32) -> float
+int to_float(float* out, const void* img, int u8, int64_t n, hipStream_t s);
+// valid blur x->y->z of a block then padded to full size (pad value 30)
+int blur_pad(float* out, const float* in, float* tmp1, float* tmp2, int w, int h, int l,
+             const Taps& t, int P, int m, hipStream_t s);
+}  // namespace ops
+
+// --------------------------------------------------------- fused sweeps
+struct SweepPlan;  // tile lists and kernel choice of one problem (rsf_sweep.hip)
+
+struct SweepArgs {
+    const float* tr;      // H (literal) or G (fast) of the current phi
+    const void* img;
+    int img_u8;
+    const float* blurI;   // fast mode: padded blur of the image
+    float* out0;          // F, or fo when out_fits
+    float* out1;          // fi when out_fits
+    int out_fits;
+    const float* phi;     // sweep 2 input
+    const float* F;       // sweep 2 force
+    float* phi_out;
+    float* tr_out;
+};
+
+// Kernel variants compiled into the library.
+struct Sweep1Variant {
+    int P;
+    int literal;
+    int tx, ty, nt;
+    const void* fn;
+    const char* name;
+};
+struct Sweep2Variant {
+    int literal;
+    int tx, ty;
+    const void* fn;
+    const char* name;
+};
+const std::vector<Sweep1Variant>& sweep1_variants();
+const std::vector<Sweep2Variant>& sweep2_variants();
+
+struct TilePlan {
+    int s1 = -1, s2 = -1;        // variant indices
+    int zc1 = 0, zc2 = 0;        // z-chunk of each sweep
+    int grid1 = 0, grid2 = 0;    // persistent grid sizes
+    std::vector<Tile> tiles1, tiles2;
+    DevBuf d_tiles1, d_tiles2, d_blocks;
+    std::vector<BlockDesc> blocks;
+    int64_t total_voxels = 0;
+};
+
+int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s2, int zc1,
+               int zc2, int num_cus);
+int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, const rsf_params* p,
+                  hipStream_t s);
+int launch_sweep2(const TilePlan& tp, const SweepArgs& a, const rsf_params* p, hipStream_t s);
+// tr = H(phi) (literal) or G(phi) (fast) on n voxels
+int launch_transform(float* tr, const float* phi, int64_t n, int literal, float eps, hipStream_t s);
+int default_sweep1(int P, int literal);
+int default_sweep2(int literal);
+
+// --------------------------------------------------------------- adjoint
+int adjoint_step(rsf_context* ctx, const TilePlan& tp, const void* image, int u8, const float* phi,
+                 const float* gout, float* gin, float* gparams, const rsf_params* p,
+                 const Derived& d, hipStream_t s);
+
+}  // namespace rsf

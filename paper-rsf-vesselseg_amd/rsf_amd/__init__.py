@@ -1,0 +1,337 @@
+"""Python view of librsf.so, the MI355X-native RSF level-set engine.
+
+Thin ctypes binding over the C ABI declared in include/rsf.h.  Device entry
+points take torch tensors on a HIP device (PyTorch-ROCm provides device memory
+and streams); host entry points take numpy arrays.  Nothing here computes: if
+librsf.so is missing, importing the binding raises.
+
+Volumes are numpy/torch arrays of shape (Z, Y, X) = (l, h, w), C order, the
+layout of the reference's X-fastest tira::volume (CUDA_RSF/kernel.cu:1010).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_DIR, "librsf.so")
+CLI_PATH = os.path.join(PKG_DIR, "rsf")
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_UNSUPPORTED = 2
+ERR_OUT_OF_MEMORY = 3
+ERR_DEVICE = 4
+ERR_IO = 5
+ERR_NONFINITE = 6
+
+DTYPE_U8 = 0
+DTYPE_F32 = 1
+MODE_FAST = 0
+MODE_LITERAL = 1
+
+
+class RSFError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"rsf status {status}: {msg}")
+        self.status = status
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("sigma", C.c_float),
+        ("lambda1", C.c_float),
+        ("lambda2", C.c_float),
+        ("nu", C.c_float),
+        ("mu", C.c_float),
+        ("dt", C.c_float),
+        ("epsilon", C.c_float),
+        ("sigma2", C.c_float),
+        ("iterations", C.c_int32),
+        ("mode", C.c_int32),
+    ]
+
+
+class Block(C.Structure):
+    _fields_ = [("offset", C.c_int64), ("w", C.c_int64), ("h", C.c_int64), ("l", C.c_int64)]
+
+
+def make_params(sigma=3.0, lambda1=1.0, lambda2=None, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0,
+                sigma2=0.0, iterations=1, mode=MODE_FAST) -> Params:
+    if lambda2 is None:
+        lambda2 = lambda1
+    return Params(sigma, lambda1, lambda2, nu, mu, dt, epsilon, sigma2, int(iterations), int(mode))
+
+
+def reference_defaults() -> Params:
+    p = Params()
+    lib().rsf_params_default(C.byref(p))
+    return p
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_P = C.c_void_p
+_I = C.c_int
+_I64 = C.c_int64
+_F = C.c_float
+_PP = C.POINTER(Params)
+_PB = C.POINTER(Block)
+_SIGS = {
+    "rsf_abi_version": (_I, []),
+    "rsf_last_error": (C.c_char_p, []),
+    "rsf_status_string": (C.c_char_p, [_I]),
+    "rsf_params_default": (None, [_PP]),
+    "rsf_taps": (_I, [_F, _P, _I, C.POINTER(_I), C.POINTER(_I)]),
+    "rsf_context_create": (_I, [_I, _P, C.POINTER(_P)]),
+    "rsf_context_destroy": (_I, [_P]),
+    "rsf_context_set_stream": (_I, [_P, _P]),
+    "rsf_workspace_size": (_I, [_PP, _I64, _I64, _I64, C.POINTER(_I64)]),
+    "rsf_step_device": (_I, [_P, _P, _I, _P, _P, _I64, _I64, _I64, _PP]),
+    "rsf_run_device": (_I, [_P, _P, _I, _P, _I64, _I64, _I64, _PP, _P]),
+    "rsf_run_host": (_I, [_P, _P, _I, _P, _I64, _I64, _I64, _PP, _P]),
+    "rsf_last_loop_ms": (_I, [_P, C.POINTER(_F)]),
+    "rsf_run_batched_device": (_I, [_P, _PB, _I, _P, _I, _P, _PP]),
+    "rsf_run_blocks_sequential_device": (_I, [_P, _PB, _I, _P, _I, _P, _PP]),
+    "rsf_autotune": (_I, [_P, _PB, _I, _I, _PP, C.c_char_p, _I]),
+    "rsf_profile_sweeps": (_I, [_P, _PB, _I, _P, _I, _P, _PP, C.POINTER(_F), C.POINTER(_F)]),
+    "rsf_step_adjoint_device": (_I, [_P, _PB, _I, _P, _I, _P, _P, _P, _P, _PP]),
+    "rsf_step_blocks_device": (_I, [_P, _PB, _I, _P, _I, _P, _P, _PP]),
+    "rsf_synth_tubes": (_I, [_P, _P, _I64, _I64, _I64, _I, _F, _F, _F, _F, C.c_uint64]),
+    "rsf_initial_phi": (_I, [_P, _P, _I64, _I64, _I64, _F, _F]),
+    "rsf_sdf_from_seed": (_I, [_P, _P, _I64, _I64, _I64, _F]),
+    "rsf_heaviside_cal": (_I, [_P, _P, _I, _I, _I]),
+    "rsf_deri_heaviside_cal": (_I, [_P, _P, _I, _I, _I]),
+    "rsf_inverse_cal": (_I, [_P, _P, _I, _I, _I]),
+    "rsf_multiplication_cal": (_I, [_P, _P, _P, _I, _I, _I]),
+    "rsf_division_cal": (_I, [_P, _P, _P, _I, _I, _I]),
+    "rsf_gradient_cal": (_I, [_P, _P, _P, _P, _I, _I, _I]),
+    "rsf_gradientmagnitude_cal": (_I, [_P, _P, _P, _P, _I, _I, _I]),
+    "rsf_division_nDIV": (_I, [_P] * 7 + [_I, _I, _I]),
+    "rsf_gradient_cal_3_input": (_I, [_P] * 6 + [_I, _I, _I]),
+    "rsf_normalized_div_plus": (_I, [_P] * 4 + [_I, _I, _I]),
+    "rsf_DIV_cal": (_I, [_P] * 4 + [_I, _I, _I]),
+    "rsf_adddevice_convolution_seperable": (_I, [_P, _P, _I, _I, _I, _F, _P, C.c_uint]),
+    "rsf_pad_border_cal": (_I, [_P, _P, _I, _I, _I, _I]),
+    "rsf_adddevice": (_I, [_P] * 5 + [_I, _I, _I, _F]),
+    "rsf_Final_PHI_cal": (_I, [_P] * 7 + [_F, _F, _F, _F, _I, _I, _I]),
+    "rsf_step_unfused_device": (_I, [_P, _P, _P, _P, _I64, _I64, _I64, _PP]),
+}
+EXPORTED = tuple(_SIGS)
+
+
+def lib():
+    """Load librsf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `make -C {PKG_DIR}` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int):
+    if status != OK:
+        raise RSFError(status, lib().rsf_last_error().decode(errors="replace"))
+
+
+def _ptr(a) -> int:
+    """Address of a numpy array or a torch tensor."""
+    if isinstance(a, np.ndarray):
+        if not a.flags.c_contiguous:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return a.data_ptr()
+    if a is None:
+        return 0
+    return int(a)
+
+
+def _sync(*arrays):
+    """Orders torch's pending work on these tensors before a library call:
+    the context runs on its own non-blocking stream."""
+    for a in arrays:
+        if hasattr(a, "is_cuda") and a.is_cuda:
+            import torch
+
+            torch.cuda.current_stream(a.device).synchronize()
+            return
+
+
+def _dtype_code(a) -> int:
+    s = str(getattr(a, "dtype", ""))
+    if s in ("uint8", "torch.uint8"):
+        return DTYPE_U8
+    if s in ("float32", "torch.float32"):
+        return DTYPE_F32
+    raise TypeError(f"image dtype must be uint8 or float32, got {s}")
+
+
+def taps(sigma: float):
+    buf = np.zeros(63, np.float32)
+    P, m = C.c_int(), C.c_int()
+    check(lib().rsf_taps(sigma, _ptr(buf), 63, C.byref(P), C.byref(m)))
+    return buf[: P.value].copy(), P.value, m.value
+
+
+# --------------------------------------------------------------- synthetic data
+def synth_tubes(shape, tubes=40, rmin=2.0, rmax=6.0, bias=20.0, noise=15.0, seed=1):
+    """Tube phantom (Z,Y,X) uint8 and its 0/1 mask."""
+    l, h, w = shape
+    img = np.empty(shape, np.uint8)
+    mask = np.empty(shape, np.uint8)
+    check(lib().rsf_synth_tubes(_ptr(img), _ptr(mask), w, h, l, int(tubes), rmin, rmax, bias, noise, int(seed)))
+    return img, mask
+
+
+def initial_phi(image: np.ndarray, blur_sigma=1.0, level=110.0):
+    image = np.ascontiguousarray(image, np.uint8)
+    l, h, w = image.shape
+    phi = np.empty(image.shape, np.float32)
+    check(lib().rsf_initial_phi(_ptr(image), _ptr(phi), w, h, l, blur_sigma, level))
+    return phi
+
+
+def sdf_from_seed(seed: np.ndarray, shift=13.0):
+    seed = np.ascontiguousarray(seed, np.float32)
+    l, h, w = seed.shape
+    phi = np.empty(seed.shape, np.float32)
+    check(lib().rsf_sdf_from_seed(_ptr(seed), _ptr(phi), w, h, l, shift))
+    return phi
+
+
+# -------------------------------------------------------------------- context
+class Context:
+    """Owns an rsf_context (one device, one stream)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        h = C.c_void_p()
+        check(lib().rsf_context_create(int(device), stream, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().rsf_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        check(lib().rsf_context_set_stream(self._h, stream))
+
+    # ---- host-pointer forms
+    def run_host(self, image: np.ndarray, phi: np.ndarray, params: Params, per_iteration=False):
+        image = np.ascontiguousarray(image)
+        phi = np.ascontiguousarray(phi, np.float32).copy()
+        l, h, w = phi.shape
+        if image.shape != phi.shape:
+            raise ValueError("image and phi shapes differ")
+        ms = np.zeros(max(params.iterations, 1), np.float32) if per_iteration else None
+        check(lib().rsf_run_host(self._h, _ptr(image), _dtype_code(image), _ptr(phi), w, h, l,
+                                 C.byref(params), _ptr(ms) if ms is not None else None))
+        return (phi, ms) if per_iteration else phi
+
+    # ---- device-pointer forms (torch tensors on the HIP device)
+    def step(self, image, phi_in, phi_out, params: Params):
+        l, h, w = phi_in.shape
+        _sync(image, phi_in, phi_out)
+        check(lib().rsf_step_device(self._h, _ptr(image), _dtype_code(image), _ptr(phi_in), _ptr(phi_out),
+                                    w, h, l, C.byref(params)))
+
+    def run(self, image, phi, params: Params):
+        l, h, w = phi.shape
+        _sync(image, phi)
+        check(lib().rsf_run_device(self._h, _ptr(image), _dtype_code(image), _ptr(phi), w, h, l,
+                                   C.byref(params), None))
+        return self.last_loop_ms()
+
+    def last_loop_ms(self) -> float:
+        ms = C.c_float()
+        check(lib().rsf_last_loop_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def run_batched(self, blocks, image, phi, params: Params):
+        arr = blocks_array(blocks)
+        _sync(image, phi)
+        check(lib().rsf_run_batched_device(self._h, arr, len(blocks), _ptr(image), _dtype_code(image),
+                                           _ptr(phi), C.byref(params)))
+        return self.last_loop_ms()
+
+    def run_sequential(self, blocks, image, phi, params: Params):
+        arr = blocks_array(blocks)
+        _sync(image, phi)
+        check(lib().rsf_run_blocks_sequential_device(self._h, arr, len(blocks), _ptr(image),
+                                                     _dtype_code(image), _ptr(phi), C.byref(params)))
+        return self.last_loop_ms()
+
+    def step_blocks(self, blocks, image, phi_in, phi_out, params: Params):
+        arr = blocks_array(blocks)
+        _sync(image, phi_in, phi_out)
+        check(lib().rsf_step_blocks_device(self._h, arr, len(blocks), _ptr(image), _dtype_code(image),
+                                           _ptr(phi_in), _ptr(phi_out), C.byref(params)))
+
+    def adjoint(self, blocks, image, phi, grad_out, grad_in, grad_params, params: Params):
+        arr = blocks_array(blocks)
+        _sync(image, phi, grad_out, grad_in, grad_params)
+        check(lib().rsf_step_adjoint_device(self._h, arr, len(blocks), _ptr(image), _dtype_code(image),
+                                            _ptr(phi), _ptr(grad_out), _ptr(grad_in), _ptr(grad_params),
+                                            C.byref(params)))
+
+    def profile_sweeps(self, blocks, image, phi, params: Params):
+        """Mean device ms of one sweep-1 and one sweep-2 launch over params.iterations
+        eager iterations (HIP events on the context stream)."""
+        arr = blocks_array(blocks)
+        a, b = C.c_float(), C.c_float()
+        _sync(image, phi)
+        check(lib().rsf_profile_sweeps(self._h, arr, len(blocks), _ptr(image), _dtype_code(image), _ptr(phi),
+                                       C.byref(params), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, blocks, image_dtype: int, params: Params) -> str:
+        arr = blocks_array(blocks)
+        buf = C.create_string_buffer(512)
+        check(lib().rsf_autotune(self._h, arr, len(blocks), int(image_dtype), C.byref(params), buf, 512))
+        return buf.value.decode()
+
+    def step_unfused(self, image_f32, phi_in, phi_out, params: Params):
+        l, h, w = phi_in.shape
+        _sync(image_f32, phi_in, phi_out)
+        check(lib().rsf_step_unfused_device(self._h, _ptr(image_f32), _ptr(phi_in), _ptr(phi_out), w, h, l,
+                                            C.byref(params)))
+
+
+def blocks_array(blocks):
+    """[(offset, w, h, l), ...] -> ctypes array of Block."""
+    arr = (Block * len(blocks))()
+    for k, (off, w, h, l) in enumerate(blocks):
+        arr[k] = Block(int(off), int(w), int(h), int(l))
+    return arr
+
+
+def pack_blocks(shapes):
+    """Consecutive packing of (l, h, w) shapes -> (blocks, total voxels)."""
+    blocks, off = [], 0
+    for (l, h, w) in shapes:
+        blocks.append((off, w, h, l))
+        off += w * h * l
+    return blocks, off

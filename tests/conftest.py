@@ -4,6 +4,11 @@ import sys
 import numpy as np
 import pytest
 
+try:  # load torch's HIP runtime first so librsf binds to the same one
+    import torch  # noqa: F401
+except Exception:  # pragma: no cover
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "paper-rsf-vesselseg_amd")
 for p in (ROOT, PKG, os.path.dirname(os.path.abspath(__file__))):

@@ -1,0 +1,173 @@
+"""GPU parity of the fused engine (sweeps, graph-captured loop, batched
+blocks) against the oracle, with the north-star gates: after one step
+max|dphi| <= 1e-4 * max|phi|; after a run the sign masks differ on <= 1e-4
+of the voxels."""
+import numpy as np
+import pytest
+
+import oracle
+import rsf_amd
+from gpu_util import mask_disagreement, one_step_gate, oracle_params, scene, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch  # noqa: F401  (binds librsf to torch's HIP runtime)
+
+    c = rsf_amd.Context(0)
+    yield c
+    c.close()
+
+
+CASES = [
+    # sigma, eps, mode, image dtype, shape (l,h,w)
+    (3.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (48, 50, 53)),
+    (3.0, 0.2, rsf_amd.MODE_FAST, np.uint8, (45, 64, 70)),
+    (3.0, 1.0, rsf_amd.MODE_LITERAL, np.uint8, (48, 50, 53)),
+    (3.0, 0.2, rsf_amd.MODE_LITERAL, np.float32, (41, 47, 66)),
+    (1.0, 1.0, rsf_amd.MODE_FAST, np.float32, (30, 33, 37)),
+    (1.0, 0.2, rsf_amd.MODE_LITERAL, np.uint8, (30, 33, 37)),
+    (5.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (40, 41, 43)),
+    (5.0, 1.0, rsf_amd.MODE_LITERAL, np.uint8, (40, 41, 43)),
+    (1.3, 1.0, rsf_amd.MODE_FAST, np.uint8, (24, 26, 28)),   # P = 9: unfused fallback
+]
+
+
+@pytest.mark.parametrize("sigma,eps,mode,dtype,shape", CASES)
+def test_fused_one_step_matches_oracle(ctx, sigma, eps, mode, dtype, shape):
+    import torch
+
+    img, phi0, _ = scene(shape, tubes=6, seed=int(sigma * 10) + mode)
+    p = rsf_amd.make_params(sigma=sigma, epsilon=eps, mode=mode, iterations=1)
+    d_img = to_dev(img.astype(dtype))
+    d_phi = to_dev(phi0)
+    d_out = torch.empty_like(d_phi)
+    ctx.step(d_img, d_phi, d_out, p)
+    torch.cuda.synchronize()
+    got = d_out.cpu().numpy()
+    ref = oracle.step(phi0, img.astype(np.float32), oracle_params(p))
+    err, tol = one_step_gate(got, ref)
+    assert err <= tol, (err, tol)
+    # tighter than the gate: both paths compute the same FP32 sums
+    bound = 2e-6 if mode == rsf_amd.MODE_LITERAL else 2e-5
+    assert err <= bound * float(np.abs(ref).max()), err
+
+
+def test_fused_lambda1_ne_lambda2_and_sigma2(ctx):
+    import torch
+
+    img, phi0, _ = scene((36, 38, 40), tubes=5, seed=3)
+    for kw in (dict(lambda1=1.3, lambda2=0.7), dict(sigma2=1.0), dict(sigma2=1.0, mode=rsf_amd.MODE_LITERAL)):
+        p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=1, **kw)
+        d_phi, d_img = to_dev(phi0), to_dev(img)
+        d_out = torch.empty_like(d_phi)
+        ctx.step(d_img, d_phi, d_out, p)
+        got = d_out.cpu().numpy()
+        ref = oracle.step(phi0, img.astype(np.float32), oracle_params(p))
+        err, tol = one_step_gate(got, ref)
+        assert err <= tol, (kw, err, tol)
+
+
+@pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
+def test_run_matches_oracle_mask(ctx, mode):
+    """Config-1 style run (128^3 is the bench's CPU case; 64^3 x 40 here)."""
+    img, phi0, _ = scene((64, 64, 64), tubes=12, seed=1)
+    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, mode=mode, iterations=40)
+    d_img, d_phi = to_dev(img), to_dev(phi0)
+    ctx.run(d_img, d_phi, p)
+    got = d_phi.cpu().numpy()
+    ref = oracle.run(phi0, img.astype(np.float32), oracle_params(p), 40)
+    assert mask_disagreement(got, ref) <= 1e-4
+    # away from the front the two agree closely; single voxels at the front can
+    # drift after many steps (the gate above is the north-star criterion)
+    diff = np.abs(got - ref)
+    assert float(np.median(diff)) <= 1e-5 * float(np.abs(ref).max())
+
+
+def test_graph_run_equals_eager_steps(ctx):
+    import torch
+
+    img, phi0, _ = scene((40, 42, 44), tubes=5, seed=4)
+    p1 = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=1)
+    d_img = to_dev(img)
+    a, b = to_dev(phi0), torch.empty_like(to_dev(phi0))
+    eager = {}
+    for t in range(1, 71):
+        ctx.step(d_img, a, b, p1)
+        a, b = b, a
+        if t in (5, 70):
+            eager[t] = a.cpu().numpy()
+    # T = 5: one 4-iteration graph + remainder; T = 70: 64-iteration graph + 6
+    for T in (5, 70):
+        pT = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=T)
+        for _ in range(2):  # the second call replays the cached graph
+            d_phi = to_dev(phi0)
+            ctx.run(d_img, d_phi, pT)
+            assert np.array_equal(d_phi.cpu().numpy(), eager[T])
+
+
+def test_per_iteration_timing_host_path(ctx):
+    img, phi0, _ = scene((40, 40, 40), tubes=4, seed=5)
+    for T in (1, 4, 7):
+        p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=T)
+        out, ms = ctx.run_host(img, phi0, p, per_iteration=True)
+        assert ms.shape == (T,) and np.all(ms > 0)
+        ref = oracle.run(phi0, img.astype(np.float32), oracle_params(p), T)
+        assert mask_disagreement(out, ref) <= 1e-4
+
+
+def test_batched_equals_sequential_blocks(ctx):
+    import torch
+
+    rng = np.random.default_rng(6)
+    shapes = [tuple(int(s) for s in rng.choice([24, 33, 40, 52], 3)) for _ in range(7)]
+    blocks, total = rsf_amd.pack_blocks(shapes)
+    img = np.empty(total, np.uint8)
+    phi = np.empty(total, np.float32)
+    for k, ((off, w, h, l), shp) in enumerate(zip(blocks, shapes)):
+        bi, bp, _ = scene(shp, tubes=3, seed=100 + k)
+        img[off:off + bi.size] = bi.ravel()
+        phi[off:off + bp.size] = bp.ravel()
+    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=6)
+    d_img = to_dev(img)
+    a, b = to_dev(phi), to_dev(phi)
+    ctx.run_batched(blocks, d_img, a, p)
+    ctx.run_sequential(blocks, d_img, b, p)
+    torch.cuda.synchronize()
+    A, B = a.cpu().numpy(), b.cpu().numpy()
+    assert np.array_equal(A, B)
+    # and each block equals the oracle on that block alone
+    for (off, w, h, l), shp in list(zip(blocks, shapes))[:3]:
+        ref = oracle.run(phi[off:off + w * h * l].reshape(shp), img[off:off + w * h * l].reshape(shp)
+                         .astype(np.float32), oracle_params(p), 6)
+        assert mask_disagreement(A[off:off + w * h * l].reshape(shp), ref) <= 1e-4
+
+
+def test_unfused_device_step_matches_oracle(ctx):
+    import torch
+
+    img, phi0, _ = scene((30, 31, 33), tubes=4, seed=7)
+    p = rsf_amd.make_params(sigma=3.0, epsilon=0.2, iterations=1, mode=rsf_amd.MODE_LITERAL)
+    out = torch.empty_like(to_dev(phi0))
+    ctx.step_unfused(to_dev(img.astype(np.float32)), to_dev(phi0), out, p)
+    ref = oracle.step(phi0, img.astype(np.float32), oracle_params(p))
+    err, _ = one_step_gate(out.cpu().numpy(), ref)
+    assert err <= 1e-6 * np.abs(ref).max()
+
+
+def test_errors_are_reported(ctx):
+    import torch
+
+    img, phi0, _ = scene((24, 24, 24), tubes=2, seed=8)
+    d_img, d_phi = to_dev(img), to_dev(phi0)
+    out = torch.empty_like(d_phi)
+    with pytest.raises(rsf_amd.RSFError) as e:
+        ctx.step(d_img, d_phi, out, rsf_amd.make_params(sigma=2.0))   # floor(7 sigma) even
+    assert e.value.status == rsf_amd.ERR_INVALID_ARGUMENT
+    with pytest.raises(rsf_amd.RSFError):
+        ctx.step(d_img, d_phi, d_phi, rsf_amd.make_params(sigma=3.0))  # aliasing
+    small = to_dev(phi0[:10, :10, :10].copy())
+    with pytest.raises(rsf_amd.RSFError):
+        ctx.step(to_dev(img[:10, :10, :10].copy()), small, torch.empty_like(small), rsf_amd.make_params(sigma=3.0))

@@ -1,0 +1,172 @@
+"""CPU-only checks of librsf.so: it loads, exports every symbol include/rsf.h
+declares, and its host-side parts (taps, phantom generator, signed distance,
+CLI argument and .npy handling) behave; no GPU compute is called here."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import rsf_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rsf.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rsf_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 38
+    L = rsf_amd.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the binding knows every one of them
+    assert sorted(set(names) - set(rsf_amd.EXPORTED)) == []
+
+
+def test_abi_version_and_status_strings():
+    L = rsf_amd.lib()
+    assert L.rsf_abi_version() == 1
+    assert L.rsf_status_string(0) == b"ok"
+    assert L.rsf_status_string(1) == b"invalid argument"
+
+
+def test_reference_defaults():
+    p = rsf_amd.reference_defaults()  # CUDA_RSF/main.cpp:91-97
+    assert (p.sigma, p.iterations, p.mu) == (5.0, 30, 1.0)
+    assert abs(p.dt - 0.1) < 1e-7 and abs(p.lambda1 - 0.1) < 1e-7 and abs(p.nu - 1e-4) < 1e-10
+    assert p.sigma2 == 0.0 and abs(p.epsilon - 0.2) < 1e-7
+
+
+@pytest.mark.parametrize("sigma", [1.0, 2.5, 3.0, 5.0])
+def test_taps_match_oracle(sigma):
+    g, P, m = rsf_amd.taps(sigma)
+    go, Po, mo = oracle.taps(sigma)
+    assert (P, m) == (Po, mo)
+    assert np.array_equal(g, go)
+
+
+@pytest.mark.parametrize("sigma", [2.0, 4.0, 0.1, -1.0])
+def test_taps_reject(sigma):
+    with pytest.raises(rsf_amd.RSFError):
+        rsf_amd.taps(sigma)
+
+
+def test_context_without_device_fails_cleanly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(rsf_amd.RSFError) as e:
+        rsf_amd.Context(0)
+    assert e.value.status == rsf_amd.ERR_DEVICE
+
+
+# ----------------------------------------------------------------- generator
+def test_synth_tubes_deterministic_and_shaped():
+    a, ma = rsf_amd.synth_tubes((40, 44, 48), tubes=6, seed=3)
+    b, mb = rsf_amd.synth_tubes((40, 44, 48), tubes=6, seed=3)
+    c, _ = rsf_amd.synth_tubes((40, 44, 48), tubes=6, seed=4)
+    assert np.array_equal(a, b) and np.array_equal(ma, mb) and not np.array_equal(a, c)
+    assert a.dtype == np.uint8 and set(np.unique(ma)) <= {0, 1} and 0 < ma.mean() < 0.5
+    # no noise, no bias: exactly 200 in the tubes and 30 outside
+    d, md = rsf_amd.synth_tubes((40, 44, 48), tubes=6, seed=3, noise=0.0, bias=0.0)
+    assert np.array_equal(md, ma)
+    assert np.all(d[md == 1] == 200) and np.all(d[md == 0] == 30)
+    # the bias is a linear x ramp of +-20
+    e, _ = rsf_amd.synth_tubes((8, 8, 41), tubes=0, noise=0.0, bias=20.0)
+    assert e[0, 0, 0] == 10 and e[0, 0, -1] == 50 and e[0, 0, 20] == 30
+    # noise sigma ~ 15 on the background
+    f, mf = rsf_amd.synth_tubes((64, 64, 64), tubes=0, noise=15.0, bias=0.0, seed=9)
+    assert abs(float(f.std()) - 15.0) < 0.3 and abs(float(f.mean()) - 30.0) < 0.2
+
+
+def test_initial_phi_is_exact_signed_distance():
+    from scipy import ndimage
+
+    img, _ = rsf_amd.synth_tubes((36, 40, 44), tubes=5, seed=5)
+    phi = rsf_amd.initial_phi(img, 1.0, 110.0)
+    m = ndimage.gaussian_filter(img.astype(np.float64), 1.0, mode="nearest", truncate=3.0) > 110
+    ref = np.where(m, -(ndimage.distance_transform_edt(m) - 0.5), ndimage.distance_transform_edt(~m) - 0.5)
+    assert np.abs(phi - ref).max() < 1e-4
+    assert np.array_equal(phi < 0, m)
+
+
+def test_sdf_from_seed_sign_and_distance():
+    from scipy import ndimage
+
+    rng = np.random.default_rng(1)
+    seed = np.zeros((20, 22, 24), np.float32)
+    seed[5:12, 6:15, 4:20] = 255.0
+    seed[15:18, 2:5, 2:5] = 14.0     # just above the shift of 13: inside
+    seed[1:3, 18:20, 20:22] = 13.0   # exactly 13: outside (seed - 13 > 0 is false)
+    phi = rsf_amd.sdf_from_seed(seed, 13.0)
+    region = seed - 13.0 > 0
+    ref = np.where(region, ndimage.distance_transform_edt(region) - 0.5,
+                   -(ndimage.distance_transform_edt(~region) - 0.5))
+    assert np.abs(phi - ref).max() < 1e-4
+    assert np.all(phi[region] > 0) and np.all(phi[~region] < 0)
+    _ = rng
+
+
+# ----------------------------------------------------------------------- CLI
+def _cli(*args, cwd=None):
+    return subprocess.run([rsf_amd.CLI_PATH, *args], capture_output=True, text=True, cwd=cwd, timeout=60)
+
+
+def test_cli_help_exits_1_like_the_reference():
+    r = _cli("--help")
+    assert r.returncode == 1  # CUDA_RSF/main.cpp:109-112
+    for opt in ("--sigma", "--t", "--dt", "--meu", "--e", "--v", "--sigma2", "--binary", "--image", "--ouput"):
+        assert opt in r.stdout
+
+
+def test_cli_rejects_unknown_and_ambiguous_options():
+    assert _cli("--nope", "1").returncode != 0
+    r = _cli("--sig", "3")  # --sigma / --sigma2
+    assert r.returncode != 0 and "ambiguous" in r.stderr
+    assert _cli("-t", "3").returncode != 0  # short options are disabled (main.cpp:104-106)
+
+
+def _save_raw_npy(path, header, payload):
+    h = header.encode()
+    total = 10 + len(h) + 1
+    h += b" " * ((64 - total % 64) % 64) + b"\n"
+    with open(path, "wb") as f:
+        f.write(b"\x93NUMPY\x01\x00" + len(h).to_bytes(2, "little") + h + payload)
+
+
+def test_cli_input_errors(tmp_path):
+    img = tmp_path / "img.npy"
+    seed = tmp_path / "seed.npy"
+    np.save(img, np.zeros((24, 24, 24), np.uint8))
+    np.save(seed, np.zeros((24, 24, 25), np.uint8))
+    r = _cli("--image", str(img), "--binary", str(seed), "--t", "1")
+    assert r.returncode != 0 and "shapes differ" in r.stderr
+    r = _cli("--image", str(tmp_path / "missing.npy"), "--binary", str(seed))
+    assert r.returncode != 0 and "cannot open" in r.stderr
+    bad = tmp_path / "bad.npy"
+    bad.write_bytes(b"not a numpy file at all")
+    assert "not a .npy" in _cli("--image", str(bad), "--binary", str(seed)).stderr
+    trunc = tmp_path / "trunc.npy"
+    _save_raw_npy(trunc, "{'descr': '|u1', 'fortran_order': False, 'shape': (24, 24, 24), }", b"\x00" * 100)
+    assert "truncated" in _cli("--image", str(trunc), "--binary", str(seed)).stderr
+    fort = tmp_path / "fort.npy"
+    np.save(fort, np.asfortranarray(np.zeros((24, 24, 24), np.uint8)))
+    assert "Fortran" in _cli("--image", str(fort), "--binary", str(seed)).stderr
+    cplx = tmp_path / "cplx.npy"
+    np.save(cplx, np.zeros((24, 24, 24), np.complex64))
+    assert "unsupported dtype" in _cli("--image", str(cplx), "--binary", str(seed)).stderr
+    flat = tmp_path / "flat.npy"
+    np.save(flat, np.zeros((24, 24), np.uint8))
+    assert "3-D" in _cli("--image", str(flat), "--binary", str(seed)).stderr
+    assert "--binary" in _cli("--image", str(img)).stderr
+    assert _cli("--image", str(img), "--binary", str(img), "--t", "2.5").returncode != 0
+    assert _cli("--image", str(img), "--binary", str(img), "--sigma", "abc").returncode != 0
