@@ -414,6 +414,7 @@ int forward64(const double* phi, const double* I, int w, int h, int l, const ora
         v->assign(n, 0.0);
     const double eps = p->epsilon;
     std::vector<double> A(n), B(n), Hc(n), t(n);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         f.H[i] = 0.5 * (1 + (2 / kPi) * std::atan(phi[i] / eps));
         f.dh[i] = (1 / kPi) * (eps / (eps * eps + phi[i] * phi[i]));
@@ -424,6 +425,7 @@ int forward64(const double* phi, const double* I, int w, int h, int l, const ora
     diff_d(f.gx.data(), phi, w, h, l, 0);
     diff_d(f.gy.data(), phi, w, h, l, 1);
     diff_d(f.gz.data(), phi, w, h, l, 2);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         f.M[i] = std::sqrt(f.gx[i] * f.gx[i] + f.gy[i] * f.gy[i] + f.gz[i] * f.gz[i]);
         double r = 1.0 / (f.M[i] + 1e-6);
@@ -449,9 +451,11 @@ int forward64(const double* phi, const double* I, int w, int h, int l, const ora
         pad_border_t(dst.data(), bl.data(), vw, vh, vl, m);
     };
     blur_pad(f.PA, A); blur_pad(f.PH, f.H); blur_pad(f.PB, B); blur_pad(f.PHc, Hc);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) { f.fo[i] = f.PA[i] / f.PH[i]; f.fi[i] = f.PB[i] / f.PHc[i]; }
     const int s = f.s;
     const double wgt = 1.0 / ((2.0 * s + 1) * (2.0 * s + 1));
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < l; z++)
         for (int y = 0; y < h; y++)
             for (int x = 0; x < w; x++) {
@@ -469,6 +473,7 @@ int forward64(const double* phi, const double* I, int w, int h, int l, const ora
                 f.Eo[c] = s1; f.Ei[c] = s2;
             }
     const double l1 = p->lambda1, l2 = p->lambda2, dt = p->dt, nu = p->nu, mu = p->mu;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         f.F[i] = l1 * f.Eo[i] - l2 * f.Ei[i];
         phi_out[i] = phi[i] - f.dh[i] * f.F[i] * dt + nu * dt * f.dh[i] * f.kap[i] +
@@ -602,6 +607,7 @@ int oracle_step_adjoint_f64(const double* phi, const double* I, int w, int h, in
     double gl1 = 0, gl2 = 0, gnu = 0, gmu = 0;
     const int s = f.s;
     const double wgt = 1.0 / ((2.0 * s + 1) * (2.0 * s + 1));
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : gl1, gl2, gnu, gmu)
     for (int z = 0; z < l; z++)
         for (int y = 0; y < h; y++)
             for (int x = 0; x < w; x++) {

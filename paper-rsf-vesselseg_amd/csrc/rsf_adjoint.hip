@@ -1,11 +1,243 @@
-// Adjoint of one RSF step (placeholder until the kernels land).
+// Adjoint (vector-Jacobian product) of one RSF step on the device.
+//
+// Forward step (CUDA_RSF/main.cpp:234-300, sigma2 = 0):
+//   phi' = phi - dt*delta*F + nu*dt*delta*kappa + mu*dt*(L - kappa)
+//   F = lambda1*(I-fo)^2 - lambda2*(I-fi)^2, fo = PA/PH, fi = PB/PHc (valid
+//   region; the pad band has fo = fi = 1), PA = blur(I*H), PH = blur(H),
+//   PB = blur(I*(1-H)), PHc = blur(1-H), H' = delta,
+//   kappa = sum_a D_a n_a, n = g/(|g|+1e-6), L = sum_a D_a g_a, g = D phi.
+// Given u = dLoss/dphi':
+//   Fbar = -dt*delta*u, fobar = -2(I-fo)*lambda1*Fbar, fibar = 2(I-fi)*lambda2*Fbar,
+//   Hbar = blur^T(QH) + I*blur^T(QA) with QH = PHbar - PHcbar, QA = PAbar - PBbar,
+//   nbar_a = D_a^T(kbar), kbar = (nu*dt*delta - mu*dt)*u, Lbar = mu*dt*u,
+//   G_a = nbar_a/(M+1e-6) - g_a*(nbar.g)/(M(M+1e-6)^2) + D_a^T(Lbar),
+//   dphi = u + delta'*(-dt*F + nu*dt*kappa)*u + delta*Hbar + sum_a D_a^T G_a.
+// The taps are symmetric, so blur^T is the same correlation applied to the
+// valid-region adjoint with zeros outside.  D^T follows the reference edge
+// rules: along x the left operand is the previous linear element (wrap), so
+// its transpose gathers from the next linear element.
 #include "rsf_internal.hpp"
 
 namespace rsf {
+namespace adj {
+namespace {
 
-int adjoint_step(rsf_context*, const TilePlan&, const void*, int, const float*, const float*, float*, float*,
-                 const rsf_params*, const Derived&, hipStream_t) {
-    return set_error(RSF_ERR_UNSUPPORTED, "adjoint not built");
+constexpr int kT = 256;
+constexpr float kEta = 1e-6f;
+
+inline dim3 grid_for(int64_t n) {
+    int64_t b = (n + kT - 1) / kT;
+    if (b > (1 << 16)) b = 1 << 16;
+    return dim3((unsigned)(b < 1 ? 1 : b));
 }
 
+#define VOXELS(i, n) \
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+__device__ __forceinline__ float dirac(float p, float eps) {
+    return (float)(1.0 / kPi) * eps / (eps * eps + p * p);
+}
+__device__ __forceinline__ float dirac_prime(float p, float eps) {
+    const float q = eps * eps + p * p;
+    return -(float)(2.0 / kPi) * eps * p / (q * q);
+}
+__device__ __forceinline__ float image_at(const void* img, int u8, int64_t i) {
+    return u8 ? (float)((const uint8_t*)img)[i] : ((const float*)img)[i];
+}
+
+// (D_a^T u)(j) for a value getter u(q) over block-local linear indices.
+template <class U>
+__device__ __forceinline__ float dT_x(const U& u, int64_t j, int x, int w, int64_t N) {
+    float plus = 0.f, minus = 0.f;
+    if (x != 0) plus += u(j - 1);
+    if (x == w - 1) plus += u(j);
+    if (j + 1 < N) minus += u(j + 1);
+    if (j == 0) minus += u(0);
+    return (plus - minus) * 0.5f;
+}
+template <class U>
+__device__ __forceinline__ float dT_yz(const U& u, int64_t j, int c, int n, int64_t stride) {
+    float plus = 0.f, minus = 0.f;
+    if (c >= 1) plus += u(j - stride);
+    if (c == n - 1) plus += u(j);
+    if (c == 0) minus += u(j);
+    if (c <= n - 2) minus += u(j + stride);
+    return (plus - minus) * 0.5f;
+}
+
+__global__ void k_pointwise_q(float* QH, float* QA, const float* phi, const float* gout, const void* img, int u8,
+                              const float* fo, const float* fi, const float* PH, const float* PHc, int w, int h,
+                              int l, int m, float dt, float l1, float l2, float eps) {
+    const int64_t plane = (int64_t)w * h, n = plane * l;
+    VOXELS(i, n) {
+        const int x = (int)(i % w), y = (int)((i / w) % h), z = (int)(i / plane);
+        const bool valid = x >= m && x < w - m && y >= m && y < h - m && z >= m && z < l - m;
+        float qh = 0.f, qa = 0.f;
+        if (valid) {
+            const float I = image_at(img, u8, i);
+            const float Fb = -dt * dirac(phi[i], eps) * gout[i];
+            const float fob = -2.f * (I - fo[i]) * l1 * Fb;
+            const float fib = 2.f * (I - fi[i]) * l2 * Fb;
+            const float ph = PH[i], phc = PHc[i];
+            qh = -fob * fo[i] / ph + fib * fi[i] / phc;   // PHbar - PHcbar
+            qa = fob / ph - fib / phc;                    // PAbar - PBbar
+        }
+        QH[i] = qh;
+        QA[i] = qa;
+    }
+}
+
+// out(x) = sum_k g[k] * in(x - m + k) along one axis, in = 0 outside the block.
+__global__ void k_conv_same_zero(float* out, const float* in, int w, int h, int l, int axis, Taps t, int P, int m) {
+    const int64_t plane = (int64_t)w * h, n = plane * l;
+    const int dim = axis == 0 ? w : (axis == 1 ? h : l);
+    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? (int64_t)w : plane);
+    VOXELS(i, n) {
+        const int c = axis == 0 ? (int)(i % w) : (axis == 1 ? (int)((i / w) % h) : (int)(i / plane));
+        float acc = 0.f;
+        for (int k = 0; k < P; ++k) {
+            const int cc = c - m + k;
+            if (cc >= 0 && cc < dim) acc = fmaf(t.g[k], in[i + (int64_t)(cc - c) * stride], acc);
+        }
+        out[i] = acc;
+    }
+}
+
+// G_a at every voxel: normalisation Jacobian applied to D^T(kbar) plus D^T(Lbar).
+__global__ void k_stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* gout, int w, int h,
+                            int l, float dt, float nu, float mu, float eps) {
+    const int64_t plane = (int64_t)w * h, n = plane * l;
+    auto kbar = [&](int64_t q) { return (nu * dt * dirac(phi[q], eps) - mu * dt) * gout[q]; };
+    auto lbar = [&](int64_t q) { return mu * dt * gout[q]; };
+    VOXELS(j, n) {
+        const int x = (int)(j % w), y = (int)((j / w) % h), z = (int)(j / plane);
+        const float nbx = dT_x(kbar, j, x, w, n), nby = dT_yz(kbar, j, y, h, w), nbz = dT_yz(kbar, j, z, l, plane);
+        const float lbx = dT_x(lbar, j, x, w, n), lby = dT_yz(lbar, j, y, h, w), lbz = dT_yz(lbar, j, z, l, plane);
+        const Grad g = grad_at(phi, x, y, z, w, h, l);
+        const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+        const float r = 1.0f / (M + kEta);
+        const float c = M > 0.f ? (nbx * g.x + nby * g.y + nbz * g.z) * r * r / M : 0.f;
+        Gx[j] = nbx * r - g.x * c + lbx;
+        Gy[j] = nby * r - g.y * c + lby;
+        Gz[j] = nbz * r - g.z * c + lbz;
+    }
+}
+
+// dphi and per-workgroup partial sums of the parameter gradients.
+__global__ __launch_bounds__(kT) void k_finish(float* gin, float* partials, const float* phi, const float* gout,
+                                               const float* Gx, const float* Gy, const float* Gz, const float* Bh,
+                                               const float* Ba, const void* img, int u8, const float* fo,
+                                               const float* fi, int w, int h, int l, float dt, float nu, float mu,
+                                               float eps, float l1, float l2) {
+    const int64_t plane = (int64_t)w * h, n = plane * l;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    auto gx = [&](int64_t q) { return Gx[q]; };
+    auto gy = [&](int64_t q) { return Gy[q]; };
+    auto gz = [&](int64_t q) { return Gz[q]; };
+    VOXELS(i, n) {
+        const int x = (int)(i % w), y = (int)((i / w) % h), z = (int)(i / plane);
+        const float st = dT_x(gx, i, x, w, n) + dT_yz(gy, i, y, h, w) + dT_yz(gz, i, z, l, plane);
+        // forward curvature and Laplacian at i (same stencil as sweep 2)
+        int qlx, qly, qlz;
+        left_x(x, y, z, w, h, qlx, qly, qlz);
+        const int rxx = (x == w - 1) ? x : x + 1;
+        const int ryy = (y == 0) ? 1 : (y == h - 1 ? y : y + 1);
+        const int lyy = (y == 0) ? 0 : (y == h - 1 ? h - 2 : y - 1);
+        const int rzz = (z == 0) ? 1 : (z == l - 1 ? z : z + 1);
+        const int lzz = (z == 0) ? 0 : (z == l - 1 ? l - 2 : z - 1);
+        const Grad gRx = grad_at(phi, rxx, y, z, w, h, l), gLx = grad_at(phi, qlx, qly, qlz, w, h, l);
+        const Grad gRy = grad_at(phi, x, ryy, z, w, h, l), gLy = grad_at(phi, x, lyy, z, w, h, l);
+        const Grad gRz = grad_at(phi, x, y, rzz, w, h, l), gLz = grad_at(phi, x, y, lzz, w, h, l);
+        auto nc = [](const Grad& g, float c) { return c / (sqrtf(g.x * g.x + g.y * g.y + g.z * g.z) + kEta); };
+        const float kap = ((nc(gRx, gRx.x) - nc(gLx, gLx.x)) + (nc(gRy, gRy.y) - nc(gLy, gLy.y)) +
+                           (nc(gRz, gRz.z) - nc(gLz, gLz.z))) * 0.5f;
+        const float lap = ((gRx.x - gLx.x) + (gRy.y - gLy.y) + (gRz.z - gLz.z)) * 0.5f;
+        const float I = image_at(img, u8, i);
+        const float d1 = I - fo[i], d2 = I - fi[i];
+        const float Eo = d1 * d1, Ei = d2 * d2;
+        const float F = l1 * Eo - l2 * Ei;
+        const float p0 = phi[i], u = gout[i];
+        const float d = dirac(p0, eps);
+        const float Fb = -dt * d * u;
+        gin[i] = u + dirac_prime(p0, eps) * (-dt * F + nu * dt * kap) * u + d * (Bh[i] + I * Ba[i]) + st;
+        s0 += Eo * Fb;
+        s1 -= Ei * Fb;
+        s2 += dt * d * kap * u;
+        s3 += dt * (lap - kap) * u;
+    }
+    __shared__ float red[4][kT];
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    red[2][threadIdx.x] = s2;
+    red[3][threadIdx.x] = s3;
+    __syncthreads();
+    for (int o = kT / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) partials[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// grad_params[q] += sum over partial rows (fixed order: deterministic).
+__global__ void k_reduce_params(float* gparams, const float* partials, int rows) {
+    const int q = threadIdx.x;
+    if (q >= 4) return;
+    double s = 0.0;
+    for (int r = 0; r < rows; ++r) s += (double)partials[4 * r + q];
+    gparams[q] += (float)s;
+}
+
+}  // namespace
+
+int pointwise_q(float* QH, float* QA, const float* phi, const float* gout, const void* img, int u8, const float* fo,
+                const float* fi, const float* PH, const float* PHc, int w, int h, int l, int m, const rsf_params* p,
+                hipStream_t s) {
+    const int64_t n = (int64_t)w * h * l;
+    hipLaunchKernelGGL(k_pointwise_q, grid_for(n), dim3(kT), 0, s, QH, QA, phi, gout, img, u8, fo, fi, PH, PHc, w, h,
+                       l, m, p->dt, p->lambda1, p->lambda2, p->epsilon);
+    RSF_HIP(hipGetLastError());
+    return RSF_OK;
+}
+
+int blur_transpose(float* out, const float* in, float* t1, float* t2, int w, int h, int l, const Derived& d,
+                   hipStream_t s) {
+    const int64_t n = (int64_t)w * h * l;
+    hipLaunchKernelGGL(k_conv_same_zero, grid_for(n), dim3(kT), 0, s, t1, in, w, h, l, 0, d.taps, d.P, d.m);
+    hipLaunchKernelGGL(k_conv_same_zero, grid_for(n), dim3(kT), 0, s, t2, (const float*)t1, w, h, l, 1, d.taps, d.P,
+                       d.m);
+    hipLaunchKernelGGL(k_conv_same_zero, grid_for(n), dim3(kT), 0, s, out, (const float*)t2, w, h, l, 2, d.taps, d.P,
+                       d.m);
+    RSF_HIP(hipGetLastError());
+    return RSF_OK;
+}
+
+int stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* gout, int w, int h, int l,
+              const rsf_params* p, hipStream_t s) {
+    const int64_t n = (int64_t)w * h * l;
+    hipLaunchKernelGGL(k_stencil_g, grid_for(n), dim3(kT), 0, s, Gx, Gy, Gz, phi, gout, w, h, l, p->dt, p->nu, p->mu,
+                       p->epsilon);
+    RSF_HIP(hipGetLastError());
+    return RSF_OK;
+}
+
+int finish_rows(int64_t n) { return (int)grid_for(n).x; }
+
+int finish(float* gin, float* partials, const float* phi, const float* gout, const float* Gx, const float* Gy,
+           const float* Gz, const float* Bh, const float* Ba, const void* img, int u8, const float* fo,
+           const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s) {
+    const int64_t n = (int64_t)w * h * l;
+    hipLaunchKernelGGL(k_finish, grid_for(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, Bh, Ba, img, u8,
+                       fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2);
+    RSF_HIP(hipGetLastError());
+    return RSF_OK;
+}
+
+int reduce_params(float* gparams, const float* partials, int rows, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_params, dim3(1), dim3(64), 0, s, gparams, partials, rows);
+    RSF_HIP(hipGetLastError());
+    return RSF_OK;
+}
+
+}  // namespace adj
 }  // namespace rsf

@@ -150,6 +150,7 @@ struct rsf_context {
     // workspace (grow-only); ws_epoch changes whenever a pointer changes
     DevBuf phi2, tr, F, blurI, imgf, tmp1, tmp2, fo, fi, Eo, Ei;
     DevBuf unf[20];
+    DevBuf adj[14];       // adjoint: fo, fi, PH, PHc, QH, QA, Bh, Ba, Gx, Gy, Gz, t1, t2, partials
     DevBuf h_img, h_phi;  // staging of rsf_run_host
     uint64_t ws_epoch = 1;
     std::list<std::unique_ptr<Runner>> runners;
@@ -878,12 +879,52 @@ int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks
     RSF_TRY(check_dtype(image_dtype));
     if (!image || !phi || !grad_out || !grad_in || !grad_params)
         return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (grad_in == grad_out || grad_in == phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "grad_in must not alias");
     Prepared pr;
     RSF_TRY(prepare(c, blocks, nblocks, p, &pr));
     if (!pr.fused) return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel", pr.d.P);
-    RSF_TRY(adjoint_step(c, pr.runner->plan, image, image_dtype == RSF_DTYPE_U8, phi, grad_out, grad_in, grad_params,
-                         p, pr.d, c->stream));
-    RSF_HIP(hipStreamSynchronize(c->stream));
+    if (pr.sw != 0) return set_error(RSF_ERR_UNSUPPORTED, "the device adjoint covers sigma2 = 0");
+    const int u8 = image_dtype == RSF_DTYPE_U8;
+    RSF_TRY(ensure_workspace(c, pr));
+    const size_t nb = (size_t)pr.extent * sizeof(float);
+    for (int k = 0; k < 13; ++k) RSF_TRY(ensure_ws(c, c->adj[k], nb));
+    int rows = 0;
+    for (auto& b : pr.blocks) rows += adj::finish_rows((int64_t)b.w * b.h * b.l);
+    RSF_TRY(ensure_ws(c, c->adj[13], (size_t)rows * 4 * sizeof(float)));
+    float* A[14];
+    for (int k = 0; k < 14; ++k) A[k] = c->adj[k].as<float>();
+    float *fo = A[0], *fi = A[1], *PH = A[2], *PHc = A[3], *QH = A[4], *QA = A[5], *Bh = A[6], *Ba = A[7];
+    float *Gx = A[8], *Gy = A[9], *Gz = A[10], *t1 = A[11], *t2 = A[12], *parts = A[13];
+    hipStream_t s = c->stream;
+    RSF_TRY(prepare_image(c, pr, image, u8));
+    RSF_TRY(launch_transform(c->tr.as<float>(), phi, pr.extent, pr.literal, p->epsilon, s));
+    SweepArgs a{};
+    a.tr = c->tr.as<float>();
+    a.img = image;
+    a.img_u8 = u8;
+    a.blurI = c->blurI.as<float>();
+    a.out0 = fo;
+    a.out1 = fi;
+    a.out2 = PH;
+    a.out3 = PHc;
+    a.out_fits = 2;
+    RSF_TRY(launch_sweep1(pr.runner->plan, a, pr.d, p, s));
+    const size_t esz = u8 ? 1 : 4;
+    int row = 0;
+    for (auto& b : pr.blocks) {
+        const int64_t o = b.off;
+        const void* im = (const char*)image + o * esz;
+        RSF_TRY(adj::pointwise_q(QH + o, QA + o, phi + o, grad_out + o, im, u8, fo + o, fi + o, PH + o, PHc + o, b.w,
+                                 b.h, b.l, pr.d.m, p, s));
+        RSF_TRY(adj::blur_transpose(Bh + o, QH + o, t1 + o, t2 + o, b.w, b.h, b.l, pr.d, s));
+        RSF_TRY(adj::blur_transpose(Ba + o, QA + o, t1 + o, t2 + o, b.w, b.h, b.l, pr.d, s));
+        RSF_TRY(adj::stencil_g(Gx + o, Gy + o, Gz + o, phi + o, grad_out + o, b.w, b.h, b.l, p, s));
+        RSF_TRY(adj::finish(grad_in + o, parts + 4 * row, phi + o, grad_out + o, Gx + o, Gy + o, Gz + o, Bh + o, Ba + o,
+                            im, u8, fo + o, fi + o, b.w, b.h, b.l, p, s));
+        row += adj::finish_rows((int64_t)b.w * b.h * b.l);
+    }
+    RSF_TRY(adj::reduce_params(grad_params, parts, row, s));
+    RSF_HIP(hipStreamSynchronize(s));
     return RSF_OK;
 }
 

@@ -87,9 +87,11 @@ struct SweepArgs {
     const void* img;
     int img_u8;
     const float* blurI;   // fast mode: padded blur of the image
-    float* out0;          // F, or fo when out_fits
-    float* out1;          // fi when out_fits
-    int out_fits;
+    float* out0;          // F (out_fits 0), else fo
+    float* out1;          // fi (out_fits 1, 2)
+    float* out2;          // blur(H) (out_fits 2)
+    float* out3;          // blur(1-H) (out_fits 2)
+    int out_fits;         // 0: force, 1: local fits, 2: fits and blurred Heavisides
     const float* phi;     // sweep 2 input
     const float* F;       // sweep 2 force
     float* phi_out;
@@ -134,8 +136,21 @@ int default_sweep1(int P, int literal);
 int default_sweep2(int literal);
 
 // --------------------------------------------------------------- adjoint
-int adjoint_step(rsf_context* ctx, const TilePlan& tp, const void* image, int u8, const float* phi,
-                 const float* gout, float* gin, float* gparams, const rsf_params* p,
-                 const Derived& d, hipStream_t s);
+// Kernels of the one-step vector-Jacobian product (rsf_adjoint.hip); each
+// works on one block (pointers already offset to the block).
+namespace adj {
+int pointwise_q(float* QH, float* QA, const float* phi, const float* gout, const void* img, int u8,
+                const float* fo, const float* fi, const float* PH, const float* PHc, int w, int h, int l,
+                int m, const rsf_params* p, hipStream_t s);
+int blur_transpose(float* out, const float* in, float* t1, float* t2, int w, int h, int l,
+                   const Derived& d, hipStream_t s);
+int stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* gout, int w, int h,
+              int l, const rsf_params* p, hipStream_t s);
+int finish_rows(int64_t n);
+int finish(float* gin, float* partials, const float* phi, const float* gout, const float* Gx,
+           const float* Gy, const float* Gz, const float* Bh, const float* Ba, const void* img, int u8,
+           const float* fo, const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s);
+int reduce_params(float* gparams, const float* partials, int rows, hipStream_t s);
+}  // namespace adj
 
 }  // namespace rsf

@@ -28,7 +28,7 @@ template <int P, bool LIT, int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) void sweep1_kernel(
     const float* __restrict__ tr, const void* __restrict__ img, int img_u8,
     const float* __restrict__ blurI, float* __restrict__ out0, float* __restrict__ out1,
-    int out_fits, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
+    float* __restrict__ out2, float* __restrict__ out3, int out_mode, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
     int ntiles, int zc, Taps taps, float halfS3, float l1, float l2, int same_lambda) {
     constexpr int CH = LIT ? 4 : 2;
     constexpr int M = (P - 1) / 2;
@@ -147,24 +147,33 @@ __global__ __launch_bounds__(NT) void sweep1_kernel(
                 const int64_t i = off + ((int64_t)zo * h + y) * w + x;
                 const float Iv = img_u8 ? (float)img8[i] : imgf[i];
                 float fo = 1.0f, fi = 1.0f;  // pad band: 30/30 (kernel.cu:313, main.cpp:277)
+                float bH = 30.0f, bHc = 30.0f;
                 if (zvalid && x >= M && x < w - M && y >= M && y < h - M) {
                     if constexpr (LIT) {
-                        fo = done[cc * CH + 0] / done[cc * CH + 1];
-                        fi = done[cc * CH + 2] / done[cc * CH + 3];
+                        bH = done[cc * CH + 1];
+                        bHc = done[cc * CH + 3];
+                        fo = done[cc * CH + 0] / bH;
+                        fi = done[cc * CH + 2] / bHc;
                     } else {
                         const float bI = blurI[i];
                         const float bIG = done[cc * CH + 0], bG = done[cc * CH + 1];
-                        fo = fmaf(0.5f, bI, bIG) / (halfS3 + bG);
-                        fi = fmaf(0.5f, bI, -bIG) / (halfS3 - bG);
+                        bH = halfS3 + bG;
+                        bHc = halfS3 - bG;
+                        fo = fmaf(0.5f, bI, bIG) / bH;
+                        fi = fmaf(0.5f, bI, -bIG) / bHc;
                     }
                 }
-                if (out_fits) {
-                    out0[i] = fo;
-                    out1[i] = fi;
-                } else {
+                if (out_mode == 0) {
                     const float d1 = Iv - fo, d2 = Iv - fi;
                     const float Eo = d1 * d1, Ei = d2 * d2;  // sigma2 = 0 (kernel.cu:44-45)
                     out0[i] = same_lambda ? l1 * (Eo - Ei) : (l1 * Eo - l2 * Ei);
+                } else {
+                    out0[i] = fo;
+                    out1[i] = fi;
+                    if (out_mode == 2) {
+                        out2[i] = bH;
+                        out3[i] = bHc;
+                    }
                 }
             }
         }
@@ -356,7 +365,8 @@ int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, cons
     const int ntiles = (int)tp.tiles1.size();
     const int same = (p->lambda1 == p->lambda2) ? 1 : 0;
     void* args[] = {(void*)&a.tr,       (void*)&a.img,     (void*)&a.img_u8, (void*)&a.blurI,
-                    (void*)&a.out0,     (void*)&a.out1,    (void*)&a.out_fits, (void*)&tp.d_blocks.p,
+                    (void*)&a.out0,     (void*)&a.out1,    (void*)&a.out2,   (void*)&a.out3,
+                    (void*)&a.out_fits, (void*)&tp.d_blocks.p,
                     (void*)&tp.d_tiles1.p, (void*)&ntiles, (void*)&tp.zc1,   (void*)&d.taps,
                     (void*)&d.halfS3,   (void*)&p->lambda1, (void*)&p->lambda2, (void*)&same};
     RSF_HIP(hipLaunchKernel(v.fn, dim3(tp.grid1), dim3(v.nt), args, 0, s));
