@@ -232,8 +232,41 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
     *s1 = default_sweep1(d.P, literal);
     *s2 = default_sweep2(literal);
     const int lmax = max_dim_l(blocks);
-    *zc1 = std::min(lmax, 128);
-    *zc2 = std::min(lmax, 64);
+    // Longest z-chunks that still give every CU a few tiles: a sweep-1 chunk
+    // re-reads P-1 halo planes, a sweep-2 chunk restarts its register pipeline.
+    auto chunk = [&](int tx, int ty, int per_cu, int zmin) {
+        int64_t cols = 0;
+        for (const auto& b : blocks) cols += (int64_t)((b.w + tx - 1) / tx) * ((b.h + ty - 1) / ty);
+        int zc = lmax;
+        while (zc > zmin && cols * ((lmax + zc - 1) / zc) < (int64_t)per_cu * c->num_cus) zc = (zc + 1) / 2;
+        return std::max(zc, 1);
+    };
+    *zc1 = chunk(sweep1_variants()[*s1].tx, sweep1_variants()[*s1].ty, 2, 4 * d.P);
+    *zc2 = std::min(chunk(sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, sweep2_variants()[*s2].ty, 4, 16),
+                    64);
+    // explicit overrides: RSF_SWEEP1 / RSF_SWEEP2 = "<variant name>:<z chunk>"
+    for (int which = 1; which <= 2; ++which) {
+        const char* env = std::getenv(which == 1 ? "RSF_SWEEP1" : "RSF_SWEEP2");
+        if (!env) continue;
+        std::string v(env);
+        const size_t colon = v.find(':');
+        const std::string name = v.substr(0, colon);
+        const int zc = colon == std::string::npos ? 0 : std::atoi(v.c_str() + colon + 1);
+        if (which == 1) {
+            const int a = find_variant1(name);
+            if (a >= 0 && sweep1_variants()[a].P == d.P && sweep1_variants()[a].literal == literal) {
+                *s1 = a;
+                if (zc > 0) *zc1 = zc;
+            }
+        } else {
+            const int b = find_variant2(name);
+            if (b >= 0 && sweep2_variants()[b].literal == literal) {
+                *s2 = b;
+                if (zc > 0) *zc2 = zc;
+            }
+        }
+    }
+    if (std::getenv("RSF_SWEEP1") || std::getenv("RSF_SWEEP2")) return;
     auto it = c->tune.find(tune_key(blocks, d.P, literal));
     if (it != c->tune.end()) {
         const int a = find_variant1(it->second.s1), b = find_variant2(it->second.s2);

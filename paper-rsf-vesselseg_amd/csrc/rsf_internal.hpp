@@ -109,7 +109,9 @@ struct Sweep1Variant {
 struct Sweep2Variant {
     int literal;
     int tx, ty;
-    const void* fn;
+    int vec;               // consecutive x voxels per thread of the interior kernel
+    const void* fn;        // whole-block kernel, or the interior kernel when shell_fn is set
+    const void* shell_fn;  // boundary-shell kernel of a split variant (else null)
     const char* name;
 };
 const std::vector<Sweep1Variant>& sweep1_variants();
@@ -119,8 +121,9 @@ struct TilePlan {
     int s1 = -1, s2 = -1;        // variant indices
     int zc1 = 0, zc2 = 0;        // z-chunk of each sweep
     int grid1 = 0, grid2 = 0;    // persistent grid sizes
-    std::vector<Tile> tiles1, tiles2;
-    DevBuf d_tiles1, d_tiles2, d_blocks;
+    int grid3 = 0;
+    std::vector<Tile> tiles1, tiles2, tiles3;  // sweep 1, sweep 2 (interior), sweep 2 shell
+    DevBuf d_tiles1, d_tiles2, d_tiles3, d_blocks;
     std::vector<BlockDesc> blocks;
     int64_t total_voxels = 0;
 };

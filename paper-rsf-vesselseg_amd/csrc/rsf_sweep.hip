@@ -24,24 +24,39 @@
 namespace rsf {
 namespace {
 
-template <int P, bool LIT, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void sweep1_kernel(
+// z-ring dispatch on the plane index modulo P (P <= 35): a switch, so the
+// compiler emits a short compare tree instead of a chain.
+#define RSF_ZCASE(S)                                                      \
+    case S:                                                               \
+        if constexpr (S < P) zpush<P, NC, (S < P ? S : 0)>(acc, v, taps, done); \
+        break;
+
+// WPE: minimum waves per SIMD the register allocation must allow (0 = none),
+// so that several workgroups share a CU and hide each other's barrier waits.
+template <int P, bool LIT, int TX, int TY, int NT, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1))) void sweep1_kernel(
     const float* __restrict__ tr, const void* __restrict__ img, int img_u8,
     const float* __restrict__ blurI, float* __restrict__ out0, float* __restrict__ out1,
-    float* __restrict__ out2, float* __restrict__ out3, int out_mode, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
-    int ntiles, int zc, Taps taps, float halfS3, float l1, float l2, int same_lambda) {
+    float* __restrict__ out2, float* __restrict__ out3, int out_mode, const BlockDesc* __restrict__ blocks,
+    const Tile* __restrict__ tiles, int ntiles, int zc, Taps taps, float halfS3, float l1, float l2,
+    int same_lambda) {
     constexpr int CH = LIT ? 4 : 2;
     constexpr int M = (P - 1) / 2;
-    constexpr int RW = TX + 2 * M;
+    constexpr int AL = (M + 3) & ~3;        // x halo rounded up to whole float4 groups
+    constexpr int SH = AL - M;              // aligned-window columns left of the needed ones
+    constexpr int RW = TX + 2 * M;          // LDS row: x0-M .. x0+TX+M-1
     constexpr int RH = TY + 2 * M;
     constexpr int RWP = (RW + 3) & ~3;
+    constexpr int QW = (TX + 2 * AL) / 4;   // float4 groups per aligned row
+    constexpr int NQ = RH * QW;             // float4 groups per plane
+    constexpr int PF = (NQ + NT - 1) / NT;  // groups per thread
     constexpr int COLS = (TX * TY) / NT;
     static_assert(COLS * NT == TX * TY, "tile must be a multiple of the block");
-    static_assert(TX % 4 == 0, "TX must be a multiple of 4");
+    static_assert(TX % 4 == 0 && P <= 35, "unsupported tile");
     constexpr int RX = 4;
     constexpr int XG = TX / RX;
     constexpr int NC = COLS * CH;
-    constexpr int WIN = (RX + P - 1 + 3) & ~3;  // x-pass window rounded to float4
+    constexpr int WIN = (RX + P - 1 + 3) & ~3;
 
     __shared__ __align__(16) float sIn[CH][RH][RWP];
     __shared__ __align__(16) float sX[CH][RH][TX];
@@ -58,9 +73,91 @@ __global__ __launch_bounds__(NT) void sweep1_kernel(
         const BlockDesc bd = blocks[tl.b];
         const int w = bd.w, h = bd.h, l = bd.l;
         const int64_t off = bd.off;
+        const int64_t plane = (int64_t)w * h;
         const int x0 = tl.x0, y0 = tl.y0, z0 = tl.z0;
         const int zend = min(z0 + zc, l);
         const int nplanes = (zend - z0) + P - 1;
+        // float4 path: rows 16-byte aligned and the aligned window inside the row
+        const bool vec = ((off | (int64_t)w) & 3) == 0 && x0 - AL >= 0 && x0 + TX + AL <= w;
+
+        // prefetch registers: tr and image of this thread's float4 groups
+        float4 pt[PF], pi[PF];
+        auto fetch = [&](int t) {
+            const int zi = min(max(z0 - M + t, 0), l - 1);
+            const int64_t zoff = off + (int64_t)zi * plane;
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const int e = tid + k * NT;
+                if (NQ % NT != 0 && e >= NQ) break;
+                const int r = e / QW, q = e - r * QW;
+                const int gy = min(max(y0 - M + r, 0), h - 1);
+                const int gx = x0 - AL + 4 * q;
+                const int64_t row = zoff + (int64_t)gy * w;
+                if (vec) {
+                    pt[k] = *reinterpret_cast<const float4*>(tr + row + gx);
+                    if (img_u8) {
+                        const uint32_t b4 = *reinterpret_cast<const uint32_t*>(img8 + row + gx);
+                        pi[k].x = __uint_as_float(b4);
+                    } else {
+                        pi[k] = *reinterpret_cast<const float4*>(imgf + row + gx);
+                    }
+                } else {
+                    float tv[4], iv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int64_t idx = row + min(max(gx + j, 0), w - 1);
+                        tv[j] = tr[idx];
+                        iv[j] = img_u8 ? (float)img8[idx] : imgf[idx];
+                    }
+                    pt[k] = make_float4(tv[0], tv[1], tv[2], tv[3]);
+                    pi[k] = make_float4(iv[0], iv[1], iv[2], iv[3]);
+                }
+            }
+        };
+        auto store = [&]() {
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const int e = tid + k * NT;
+                if (NQ % NT != 0 && e >= NQ) break;
+                const int r = e / QW, q = e - r * QW;
+                float iv[4];
+                if (vec && img_u8) {
+                    const uint32_t b4 = __float_as_uint(pi[k].x);
+                    iv[0] = (float)(b4 & 0xff);
+                    iv[1] = (float)((b4 >> 8) & 0xff);
+                    iv[2] = (float)((b4 >> 16) & 0xff);
+                    iv[3] = (float)(b4 >> 24);
+                } else {
+                    iv[0] = pi[k].x; iv[1] = pi[k].y; iv[2] = pi[k].z; iv[3] = pi[k].w;
+                }
+                const float tv[4] = {pt[k].x, pt[k].y, pt[k].z, pt[k].w};
+                float cv[CH][4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    cv[0][j] = iv[j] * tv[j];          // I*H (main.cpp:238) or I*G
+                    cv[1][j] = tv[j];                  // H or G
+                    if constexpr (LIT) {
+                        const float hc = 1.0f - tv[j];  // kernel.cu:295
+                        cv[2][j] = iv[j] * hc;          // I*(1-H) (main.cpp:246)
+                        cv[3][j] = hc;
+                    }
+                }
+                const int c0 = 4 * q - SH;             // LDS column of the group's first element
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) {
+                    if constexpr (SH % 2 == 0) {
+                        if (c0 >= 0 && c0 + 1 < RW)
+                            *reinterpret_cast<float2*>(&sIn[ch][r][c0]) = make_float2(cv[ch][0], cv[ch][1]);
+                        if (c0 + 2 >= 0 && c0 + 3 < RW)
+                            *reinterpret_cast<float2*>(&sIn[ch][r][c0 + 2]) = make_float2(cv[ch][2], cv[ch][3]);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (c0 + j >= 0 && c0 + j < RW) sIn[ch][r][c0 + j] = cv[ch][j];
+                    }
+                }
+            }
+        };
 
         float acc[NC][P];
 #pragma unroll
@@ -68,30 +165,24 @@ __global__ __launch_bounds__(NT) void sweep1_kernel(
 #pragma unroll
             for (int k = 0; k < P; ++k) acc[n][k] = 0.f;
 
+        fetch(0);
         for (int t = 0; t < nplanes; ++t) {
-            const int zi = min(max(z0 - M + t, 0), l - 1);
-            const int64_t zoff = off + (int64_t)zi * h * w;
-            // ---- stage A: transformed level set and image -> channels in LDS
-            for (int e = tid; e < RH * RW; e += NT) {
-                const int r = e / RW, c = e - r * RW;
-                const int gx = min(max(x0 - M + c, 0), w - 1);
-                const int gy = min(max(y0 - M + r, 0), h - 1);
-                const int64_t idx = zoff + (int64_t)gy * w + gx;
-                const float tv = tr[idx];
-                const float iv = img_u8 ? (float)img8[idx] : imgf[idx];
-                if constexpr (LIT) {
-                    const float hc = 1.0f - tv;        // kernel.cu:295
-                    sIn[0][r][c] = iv * tv;            // I*H      (main.cpp:238)
-                    sIn[1][r][c] = tv;                 // H
-                    sIn[2][r][c] = iv * hc;            // I*(1-H)  (main.cpp:246)
-                    sIn[3][r][c] = hc;                 // 1-H
-                } else {
-                    sIn[0][r][c] = iv * tv;            // I*G
-                    sIn[1][r][c] = tv;                 // G
+            store();
+            if (t + 1 < nplanes) fetch(t + 1);
+            // emission operands of this plane, loaded early
+            const int zo = z0 + t - (P - 1);
+            float eI[COLS], eB[COLS];
+            if (t >= P - 1) {
+#pragma unroll
+                for (int cc = 0; cc < COLS; ++cc) {
+                    const int x = min(x0 + tx, w - 1), y = min(y0 + ty0 + cc, h - 1);
+                    const int64_t i = off + (int64_t)zo * plane + (int64_t)y * w + x;
+                    eI[cc] = img_u8 ? (float)img8[i] : imgf[i];
+                    eB[cc] = LIT ? 0.f : blurI[i];
                 }
             }
             __syncthreads();
-            // ---- stage B: x pass, RX consecutive outputs per work item
+            // ---- x pass, RX consecutive outputs per work item
             for (int e = tid; e < CH * RH * XG; e += NT) {
                 const int ch = e / (RH * XG);
                 const int rem = e - ch * (RH * XG);
@@ -101,11 +192,11 @@ __global__ __launch_bounds__(NT) void sweep1_kernel(
                 const float4* src = reinterpret_cast<const float4*>(&sIn[ch][r][xb]);
 #pragma unroll
                 for (int q = 0; q < WIN / 4; ++q) {
-                    const float4 v = src[q];
-                    win[4 * q + 0] = v.x;
-                    win[4 * q + 1] = v.y;
-                    win[4 * q + 2] = v.z;
-                    win[4 * q + 3] = v.w;
+                    const float4 v4 = src[q];
+                    win[4 * q + 0] = v4.x;
+                    win[4 * q + 1] = v4.y;
+                    win[4 * q + 2] = v4.z;
+                    win[4 * q + 3] = v4.w;
                 }
                 float o4[RX];
 #pragma unroll
@@ -118,7 +209,7 @@ __global__ __launch_bounds__(NT) void sweep1_kernel(
                 *reinterpret_cast<float4*>(&sX[ch][r][xb]) = make_float4(o4[0], o4[1], o4[2], o4[3]);
             }
             __syncthreads();
-            // ---- stage C: y pass for this thread's COLS rows
+            // ---- y pass for this thread's COLS rows
             float v[NC];
 #pragma unroll
             for (int ch = 0; ch < CH; ++ch) {
@@ -133,19 +224,25 @@ __global__ __launch_bounds__(NT) void sweep1_kernel(
                     v[cc * CH + ch] = a;
                 }
             }
-            // ---- stage D: z ring
+            // ---- z ring
             float done[NC];
-            zdispatch<P, NC>(t % P, acc, v, taps, done, std::make_integer_sequence<int, P>{});
+            switch (t % P) {
+                RSF_ZCASE(0) RSF_ZCASE(1) RSF_ZCASE(2) RSF_ZCASE(3) RSF_ZCASE(4) RSF_ZCASE(5) RSF_ZCASE(6)
+                RSF_ZCASE(7) RSF_ZCASE(8) RSF_ZCASE(9) RSF_ZCASE(10) RSF_ZCASE(11) RSF_ZCASE(12) RSF_ZCASE(13)
+                RSF_ZCASE(14) RSF_ZCASE(15) RSF_ZCASE(16) RSF_ZCASE(17) RSF_ZCASE(18) RSF_ZCASE(19) RSF_ZCASE(20)
+                RSF_ZCASE(21) RSF_ZCASE(22) RSF_ZCASE(23) RSF_ZCASE(24) RSF_ZCASE(25) RSF_ZCASE(26) RSF_ZCASE(27)
+                RSF_ZCASE(28) RSF_ZCASE(29) RSF_ZCASE(30) RSF_ZCASE(31) RSF_ZCASE(32) RSF_ZCASE(33) RSF_ZCASE(34)
+                default: break;
+            }
             if (t < P - 1) continue;
             // ---- emission of output plane zo
-            const int zo = z0 + t - (P - 1);
             const bool zvalid = zo >= M && zo < l - M;
 #pragma unroll
             for (int cc = 0; cc < COLS; ++cc) {
                 const int x = x0 + tx, y = y0 + ty0 + cc;
                 if (x >= w || y >= h) continue;
-                const int64_t i = off + ((int64_t)zo * h + y) * w + x;
-                const float Iv = img_u8 ? (float)img8[i] : imgf[i];
+                const int64_t i = off + (int64_t)zo * plane + (int64_t)y * w + x;
+                const float Iv = eI[cc];
                 float fo = 1.0f, fi = 1.0f;  // pad band: 30/30 (kernel.cu:313, main.cpp:277)
                 float bH = 30.0f, bHc = 30.0f;
                 if (zvalid && x >= M && x < w - M && y >= M && y < h - M) {
@@ -155,12 +252,11 @@ __global__ __launch_bounds__(NT) void sweep1_kernel(
                         fo = done[cc * CH + 0] / bH;
                         fi = done[cc * CH + 2] / bHc;
                     } else {
-                        const float bI = blurI[i];
                         const float bIG = done[cc * CH + 0], bG = done[cc * CH + 1];
                         bH = halfS3 + bG;
                         bHc = halfS3 - bG;
-                        fo = fmaf(0.5f, bI, bIG) / bH;
-                        fi = fmaf(0.5f, bI, -bIG) / bHc;
+                        fo = fmaf(0.5f, eB[cc], bIG) * __builtin_amdgcn_rcpf(bH);
+                        fi = fmaf(0.5f, eB[cc], -bIG) * __builtin_amdgcn_rcpf(bHc);
                     }
                 }
                 if (out_mode == 0) {
@@ -178,6 +274,204 @@ __global__ __launch_bounds__(NT) void sweep1_kernel(
             }
         }
         __syncthreads();
+    }
+}
+#undef RSF_ZCASE
+
+// One voxel of sweep 2 with the full reference edge rules (boundary shell).
+template <bool LIT>
+__device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const float* __restrict__ F,
+                                             float* __restrict__ phi_out, float* __restrict__ tr_out, int x,
+                                             int y, int z, int w, int h, int l, float dt, float nu, float mu,
+                                             float eps, float inv_eps) {
+    const int64_t i = ((int64_t)z * h + y) * w + x;
+    int qlx, qly, qlz;
+    left_x(x, y, z, w, h, qlx, qly, qlz);
+    const int rxx = (x == w - 1) ? x : x + 1;
+    const int ryy = (y == 0) ? 1 : (y == h - 1 ? y : y + 1);
+    const int lyy = (y == 0) ? 0 : (y == h - 1 ? h - 2 : y - 1);
+    const int rzz = (z == 0) ? 1 : (z == l - 1 ? z : z + 1);
+    const int lzz = (z == 0) ? 0 : (z == l - 1 ? l - 2 : z - 1);
+    const Grad gRx = grad_at(f, rxx, y, z, w, h, l);
+    const Grad gLx = grad_at(f, qlx, qly, qlz, w, h, l);
+    const Grad gRy = grad_at(f, x, ryy, z, w, h, l);
+    const Grad gLy = grad_at(f, x, lyy, z, w, h, l);
+    const Grad gRz = grad_at(f, x, y, rzz, w, h, l);
+    const Grad gLz = grad_at(f, x, y, lzz, w, h, l);
+    float nRx, nLx, nRy, nLy, nRz, nLz;
+    if constexpr (LIT) {
+        auto nrm = [](const Grad& g, float c) {
+            const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+            return (float)((double)c / ((double)M + 1e-6));
+        };
+        nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
+        nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
+        nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
+    } else {
+        auto nrm = [](const Grad& g, float c) {
+            const float M = __builtin_amdgcn_sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+            return c * __builtin_amdgcn_rcpf(M + 1e-6f);
+        };
+        nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
+        nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
+        nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
+    }
+    const float kap = ((nRx - nLx) * 0.5f + (nRy - nLy) * 0.5f) + (nRz - nLz) * 0.5f;
+    const float lap = ((gRx.x - gLx.x) * 0.5f + (gRy.y - gLy.y) * 0.5f) + (gRz.z - gLz.z) * 0.5f;
+    const float p0 = f[i];
+    const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
+    const float p1 = ((p0 - (d * F[i]) * dt) + ((nu * dt) * d) * kap) + mu * (dt * (lap - kap));
+    phi_out[i] = p1;
+    tr_out[i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+}
+
+// Sweep 2, boundary shell: voxels within 2 of a face of their block (and,
+// next to the vector interior kernel, the columns x = 2, 3, w-4, w-3).  Work
+// items (block, first shell index, count, columns) enumerate, per block, the 4
+// planes z in {0,1,l-2,l-1}, then the rows y in {0,1,h-2,h-1} of the inner
+// planes, then `cols` columns at the two x ends of the inner rows.
+template <bool LIT>
+__global__ __launch_bounds__(256) void sweep2_shell_kernel(
+    const float* __restrict__ phi_all, const float* __restrict__ F, float* __restrict__ phi_out,
+    float* __restrict__ tr_out, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ items,
+    int nitems, float dt, float nu, float mu, float eps, float inv_eps) {
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int cols = items[it].z0;  // boundary columns per inner row: 4 or 8
+        const Tile t = items[it];
+        if (t.b < 0 || (int)threadIdx.x >= t.y0) continue;
+        const BlockDesc bd = blocks[t.b];
+        const int w = bd.w, h = bd.h, l = bd.l;
+        int64_t k = (int64_t)t.x0 + threadIdx.x;
+        const int64_t cA = 4 * (int64_t)w * h, cB = 4 * (int64_t)w * (l - 4);
+        int x, y, z;
+        if (k < cA) {
+            const int pz = (int)(k / ((int64_t)w * h));
+            const int64_t r = k - (int64_t)pz * w * h;
+            z = pz < 2 ? pz : l - 4 + pz;
+            y = (int)(r / w);
+            x = (int)(r - (int64_t)y * w);
+        } else if (k < cA + cB) {
+            k -= cA;
+            z = 2 + (int)(k / (4 * (int64_t)w));
+            const int r = (int)(k - (int64_t)(z - 2) * 4 * w);
+            const int yy = r / w;
+            y = yy < 2 ? yy : h - 4 + yy;
+            x = r - yy * w;
+        } else {
+            k -= cA + cB;
+            const int64_t per = (int64_t)cols * (h - 4);
+            z = 2 + (int)(k / per);
+            const int r = (int)(k - (int64_t)(z - 2) * per);
+            y = 2 + r / cols;
+            const int xx = r % cols, half = cols / 2;
+            x = xx < half ? xx : w - cols + xx;
+        }
+        sweep2_voxel<LIT>(phi_all + bd.off, F + bd.off, phi_out + bd.off, tr_out + bd.off, x, y, z, w, h, l, dt, nu,
+                          mu, eps, inv_eps);
+    }
+}
+
+// Sweep 2, interior: tiles cover x in [2,w-2), y in [2,h-2), z in [2,l-2) of
+// each block; each thread marches one column through z.  The radius-2 stencil
+// is read through a buffer resource whose base sits 2P+2W+2 elements before
+// the block, so every tap has a non-negative byte offset: voffset (4i) + an
+// SGPR for the y/z part + an immediate for the x part.
+template <bool LIT, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
+    const float* __restrict__ phi_all, const float* __restrict__ F, float* __restrict__ phi_out,
+    float* __restrict__ tr_out, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
+    int ntiles, int zc, float dt, float nu, float mu, float eps, float inv_eps) {
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
+        const Tile tl = tiles[item];
+        if (tl.b < 0) continue;
+        const BlockDesc bd = blocks[tl.b];
+        const int w = bd.w, h = bd.h, l = bd.l;
+        const int x = tl.x0 + tx, y = tl.y0 + ty;
+        if (x >= w - 2 || y >= h - 2) continue;
+        const int64_t W = w, P = (int64_t)w * h;
+        const int64_t shift = 2 * P + 2 * W + 2;
+        const float* f = phi_all + bd.off;
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(f - shift), 0, (int)0xffffffff, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rF =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(F + bd.off), 0, (int)0xffffffff, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rO =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(phi_out + bd.off), 0, (int)0xffffffff, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rT =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(tr_out + bd.off), 0, (int)0xffffffff, 0x00020000);
+        auto S = [&](int ky, int kz) { return (int)(uint32_t)(((2 + ky) * W + (2 + kz) * P) * 4); };
+        const int s00 = S(0, 0), sp10 = S(1, 0), sm10 = S(-1, 0), sp20 = S(2, 0), sm20 = S(-2, 0);
+        const int s0p1 = S(0, 1), s0m1 = S(0, -1), s0p2 = S(0, 2), s0m2 = S(0, -2);
+        const int spp = S(1, 1), spm = S(1, -1), smp = S(-1, 1), smm = S(-1, -1);
+        const int zend = min(tl.z0 + zc, l - 2);
+        const uint32_t vstep = (uint32_t)(P * 4);
+        // taps: 0 c, 1 x+2, 2 x-2, 3 y+2, 4 y-2, 5 z+2, 6 z-2, 7 x+y+, 8 x+y-, 9 x-y+, 10 x-y-,
+        // 11 x+z+, 12 x+z-, 13 x-z+, 14 x-z-, 15 y+z+, 16 y+z-, 17 y-z+, 18 y-z-, 19 F
+        auto load = [&](float (&r)[20], uint32_t v) {
+            auto L = [&](int kx, int so) {
+                return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(v + 4 * (kx + 2)), so, 0));
+            };
+            r[0] = L(0, s00);
+            r[1] = L(2, s00); r[2] = L(-2, s00);
+            r[3] = L(0, sp20); r[4] = L(0, sm20);
+            r[5] = L(0, s0p2); r[6] = L(0, s0m2);
+            r[7] = L(1, sp10); r[8] = L(1, sm10); r[9] = L(-1, sp10); r[10] = L(-1, sm10);
+            r[11] = L(1, s0p1); r[12] = L(1, s0m1); r[13] = L(-1, s0p1); r[14] = L(-1, s0m1);
+            r[15] = L(0, spp); r[16] = L(0, spm); r[17] = L(0, smp); r[18] = L(0, smm);
+            r[19] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rF, (int)v, 0, 0));
+        };
+        auto compute = [&](const float (&r)[20], uint32_t v) {
+            const float c0 = r[0];
+            // gradients at the six D-neighbours (interior rules)
+            const Grad gRx{(r[1] - c0) * 0.5f, (r[7] - r[8]) * 0.5f, (r[11] - r[12]) * 0.5f};
+            const Grad gLx{(c0 - r[2]) * 0.5f, (r[9] - r[10]) * 0.5f, (r[13] - r[14]) * 0.5f};
+            const Grad gRy{(r[7] - r[9]) * 0.5f, (r[3] - c0) * 0.5f, (r[15] - r[16]) * 0.5f};
+            const Grad gLy{(r[8] - r[10]) * 0.5f, (c0 - r[4]) * 0.5f, (r[17] - r[18]) * 0.5f};
+            const Grad gRz{(r[11] - r[13]) * 0.5f, (r[15] - r[17]) * 0.5f, (r[5] - c0) * 0.5f};
+            const Grad gLz{(r[12] - r[14]) * 0.5f, (r[16] - r[18]) * 0.5f, (c0 - r[6]) * 0.5f};
+            float nRx, nLx, nRy, nLy, nRz, nLz;
+            if constexpr (LIT) {
+                auto nrm = [](const Grad& g, float cc) {
+                    const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    return (float)((double)cc / ((double)M + 1e-6));
+                };
+                nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
+                nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
+                nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
+            } else {
+                auto nrm = [](const Grad& g, float cc) {
+                    const float M = __builtin_amdgcn_sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    return cc * __builtin_amdgcn_rcpf(M + 1e-6f);
+                };
+                nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
+                nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
+                nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
+            }
+            const float kap = ((nRx - nLx) * 0.5f + (nRy - nLy) * 0.5f) + (nRz - nLz) * 0.5f;
+            const float lap = ((gRx.x - gLx.x) * 0.5f + (gRy.y - gLy.y) * 0.5f) + (gRz.z - gLz.z) * 0.5f;
+            const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
+            const float p1 = ((c0 - (d * r[19]) * dt) + ((nu * dt) * d) * kap) + mu * (dt * (lap - kap));
+            const float t1 = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, p1), rO, (int)v, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, t1), rT, (int)v, 0, 0);
+        };
+        // two register sets: the next plane's loads are in flight while this one computes
+        float A[20], B[20];
+        int z = tl.z0;
+        uint32_t v = (uint32_t)((((int64_t)tl.z0 * h + y) * W + x) * 4);
+        if (z < zend) load(A, v);
+        while (z < zend) {
+            if (z + 1 < zend) load(B, v + vstep);
+            compute(A, v);
+            ++z;
+            v += vstep;
+            if (z >= zend) break;
+            if (z + 1 < zend) load(A, v + vstep);
+            compute(B, v);
+            ++z;
+            v += vstep;
+        }
     }
 }
 
@@ -206,8 +500,68 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
         const int rxx = (x == w - 1) ? x : x + 1;
         const int ryy = (y == 0) ? 1 : (y == h - 1 ? y : y + 1);
         const int lyy = (y == 0) ? 0 : (y == h - 1 ? h - 2 : y - 1);
+        const bool xy_inner = x >= 2 && x <= w - 3 && y >= 2 && y <= h - 3;
+        const int64_t P = (int64_t)w * h;
+        // Interior voxels read their radius-2 stencil through a buffer resource
+        // whose base sits 2P+2W+2 elements before the block: every tap then has
+        // a non-negative byte offset = voffset (4i) + SGPR (y, z part) + imm (x part).
+        const int64_t shift = 2 * P + 2 * (int64_t)w + 2;
+        const bool buf_ok = ((int64_t)l * P + 2 * shift) * 4 < 0xffffffffLL;
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(f - shift), 0, (int)0xffffffff, 0x00020000);
+        auto S = [&](int ky, int kz) { return (uint32_t)(((2 + ky) * (int64_t)w + (2 + kz) * P) * 4); };
+        const uint32_t s00 = S(0, 0), sp10 = S(1, 0), sm10 = S(-1, 0), sp20 = S(2, 0), sm20 = S(-2, 0);
+        const uint32_t s0p1 = S(0, 1), s0m1 = S(0, -1), s0p2 = S(0, 2), s0m2 = S(0, -2);
+        const uint32_t spp = S(1, 1), spm = S(1, -1), smp = S(-1, 1), smm = S(-1, -1);
         for (int z = tl.z0; z < zend; ++z) {
             const int64_t i = ((int64_t)z * h + y) * w + x;
+            if (buf_ok && xy_inner && z >= 2 && z <= l - 3) {
+                const uint32_t v = (uint32_t)i * 4u;
+                auto L = [&](int kx, uint32_t so) {
+                    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(v + 4 * (kx + 2)),
+                                                                                           (int)so, 0));
+                };
+                const float c0 = L(0, s00);
+                const float xp2 = L(2, s00), xm2 = L(-2, s00);
+                const float yp2 = L(0, sp20), ym2 = L(0, sm20);
+                const float zp2 = L(0, s0p2), zm2 = L(0, s0m2);
+                const float xpyp = L(1, sp10), xpym = L(1, sm10), xmyp = L(-1, sp10), xmym = L(-1, sm10);
+                const float xpzp = L(1, s0p1), xpzm = L(1, s0m1), xmzp = L(-1, s0p1), xmzm = L(-1, s0m1);
+                const float ypzp = L(0, spp), ypzm = L(0, spm), ymzp = L(0, smp), ymzm = L(0, smm);
+                // gradients at the six D-neighbours
+                const Grad gRx{(xp2 - c0) * 0.5f, (xpyp - xpym) * 0.5f, (xpzp - xpzm) * 0.5f};
+                const Grad gLx{(c0 - xm2) * 0.5f, (xmyp - xmym) * 0.5f, (xmzp - xmzm) * 0.5f};
+                const Grad gRy{(xpyp - xmyp) * 0.5f, (yp2 - c0) * 0.5f, (ypzp - ypzm) * 0.5f};
+                const Grad gLy{(xpym - xmym) * 0.5f, (c0 - ym2) * 0.5f, (ymzp - ymzm) * 0.5f};
+                const Grad gRz{(xpzp - xmzp) * 0.5f, (ypzp - ymzp) * 0.5f, (zp2 - c0) * 0.5f};
+                const Grad gLz{(xpzm - xmzm) * 0.5f, (ypzm - ymzm) * 0.5f, (c0 - zm2) * 0.5f};
+                float nRx, nLx, nRy, nLy, nRz, nLz;
+                if constexpr (LIT) {
+                    auto nrm = [](const Grad& g, float cc) {
+                        const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                        return (float)((double)cc / ((double)M + 1e-6));
+                    };
+                    nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
+                    nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
+                    nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
+                } else {
+                    auto nrm = [](const Grad& g, float cc) {
+                        const float M = __builtin_amdgcn_sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                        return cc * __builtin_amdgcn_rcpf(M + 1e-6f);
+                    };
+                    nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
+                    nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
+                    nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
+                }
+                const float kap = ((nRx - nLx) * 0.5f + (nRy - nLy) * 0.5f) + (nRz - nLz) * 0.5f;
+                const float lap = ((gRx.x - gLx.x) * 0.5f + (gRy.y - gLy.y) * 0.5f) + (gRz.z - gLz.z) * 0.5f;
+                const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
+                const float Fv = F[bd.off + i];
+                const float p1 = ((c0 - (d * Fv) * dt) + ((nu * dt) * d) * kap) + mu * (dt * (lap - kap));
+                phi_out[bd.off + i] = p1;
+                tr_out[bd.off + i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+                continue;
+            }
             int qlx = lxx, qly = lxy, qlz = z + lxz_off;
             if (qlz < 0) { qlx = 0; qly = 0; qlz = 0; }  // voxel (0,0,0): clamp
             const int rzz = (z == 0) ? 1 : (z == l - 1 ? z : z + 1);
@@ -248,27 +602,189 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
     }
 }
 
+// Sweep 2, LDS z-march.  Per gradient plane s the workgroup keeps phi planes
+// s-1, s, s+1 of its tile (+2 halo) in an LDS ring, computes g = D phi and
+// n = g/(|g|+1e-6) once per voxel of the tile (+1 halo) into LDS, keeps the
+// z components of the own column in registers, and then emits output plane
+// s-1 from the gradient planes s-2, s-1, s.  Edge rules as the reference:
+// one-sided halves at the y/z ends and at x = w-1; at x = 0 the left operand
+// is the previous linear element, so the x = -1 halo column holds phi (and g,
+// n) of that element.
+template <bool LIT, int TX, int TY, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep2_lds_kernel(
+    const float* __restrict__ phi_all, const float* __restrict__ F, float* __restrict__ phi_out,
+    float* __restrict__ tr_out, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
+    int ntiles, int zc, float dt, float nu, float mu, float eps, float inv_eps) {
+    constexpr int NT = TX * TY;
+    constexpr int PW = TX + 4, PH = TY + 4;   // phi tile with 2-voxel halo
+    constexpr int GW = TX + 2, GH = TY + 2;   // gradient tile with 1-voxel halo
+    __shared__ float sp[3][PH][PW];
+    __shared__ float sgx[2][GH][GW], sgy[2][GH][GW], snx[2][GH][GW], sny[2][GH][GW];
+    __shared__ float snz[TY][TX], sgz[TY][TX];  // z components of the interior, current plane
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+
+    for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
+        const Tile tl = tiles[item];
+        if (tl.b < 0) continue;
+        const BlockDesc bd = blocks[tl.b];
+        const int w = bd.w, h = bd.h, l = bd.l;
+        const int64_t plane = (int64_t)w * h;
+        const float* f = phi_all + bd.off;
+        const int x0 = tl.x0, y0 = tl.y0, z0 = tl.z0;
+        const int zend = min(z0 + zc, l);
+        const int s_begin = max(z0 - 1, 0), s_last = min(zend, l - 1);
+
+        // phi plane q into ring slot q % 3 (clamped reads; x = -1 holds the wrap)
+        auto load_plane = [&](int q) {
+            float(*dst)[PW] = sp[q % 3];
+            for (int e = tid; e < PH * PW; e += NT) {
+                const int r = e / PW, c = e - r * PW;
+                const int gy = min(max(y0 - 2 + r, 0), h - 1);
+                const int gx = x0 - 2 + c;
+                float v;
+                if (gx == -1 && y0 - 2 + r >= 0 && y0 - 2 + r < h) {
+                    // previous linear element of (0, gy, q)
+                    const int64_t lin = (int64_t)q * plane + (int64_t)gy * w - 1;
+                    v = f[lin < 0 ? 0 : lin];
+                } else {
+                    v = f[(int64_t)q * plane + (int64_t)gy * w + min(max(gx, 0), w - 1)];
+                }
+                dst[r][c] = v;
+            }
+        };
+        float nz_m2 = 0.f, gz_m2 = 0.f, nz_m1 = 0.f, gz_m1 = 0.f, nz_0 = 0.f, gz_0 = 0.f;
+
+        // emission of output plane z from gradient planes (buffer zb = z&1 holds plane z)
+        auto emit = [&](int z, float nzR, float gzR, float nzL, float gzL) {
+            const int x = x0 + tx, y = y0 + ty;
+            if (x >= w || y >= h) return;
+            const int b = z & 1;
+            const int cx = tx + 1, cy = ty + 1;
+            const int cxr = (x == w - 1) ? cx : cx + 1, cxl = cx - 1;
+            const int cyr = (y == 0) ? cy + 1 : (y == h - 1 ? cy : cy + 1);
+            const int cyl = (y == 0) ? cy : (y == h - 1 ? cy - 1 : cy - 1);
+            const float kap = ((snx[b][cy][cxr] - snx[b][cy][cxl]) * 0.5f + (sny[b][cyr][cx] - sny[b][cyl][cx]) * 0.5f) +
+                              (nzR - nzL) * 0.5f;
+            const float lap = ((sgx[b][cy][cxr] - sgx[b][cy][cxl]) * 0.5f + (sgy[b][cyr][cx] - sgy[b][cyl][cx]) * 0.5f) +
+                              (gzR - gzL) * 0.5f;
+            const float p0 = sp[z % 3][ty + 2][tx + 2];
+            const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
+            const int64_t i = bd.off + (int64_t)z * plane + (int64_t)y * w + x;
+            const float p1 = ((p0 - (d * F[i]) * dt) + ((nu * dt) * d) * kap) + mu * (dt * (lap - kap));
+            phi_out[i] = p1;
+            tr_out[i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+        };
+
+        load_plane(s_begin);
+        if (s_begin + 1 <= l - 1) load_plane(s_begin + 1);
+        for (int s = s_begin; s <= s_last; ++s) {
+            if (s == s_begin && s > 0) load_plane(s - 1);
+            if (s + 1 <= l - 1 && s + 1 > s_begin + 1) load_plane(s + 1);
+            __syncthreads();
+            // ---- gradient plane s on the tile + 1 halo
+            const int zr = (s == 0) ? 1 : (s == l - 1 ? s : s + 1);
+            const int zl = (s == 0) ? 0 : (s == l - 1 ? l - 2 : s - 1);
+            const int gb = s & 1;
+            for (int e = tid; e < GH * GW; e += NT) {
+                const int r = e / GW, c = e - r * GW;
+                const int x = x0 - 1 + c, y = y0 - 1 + r;
+                Grad g;
+                if (x < 0 || x >= w || y < 0 || y >= h) {
+                    if (x == -1 && y >= 0 && y < h) {
+                        // D_x left operand of (0, y, s): the previous linear element
+                        int qx, qy, qz;
+                        left_x(0, y, s, w, h, qx, qy, qz);
+                        g = grad_at(f, qx, qy, qz, w, h, l);
+                    } else {
+                        g.x = g.y = g.z = 0.f;
+                    }
+                } else {
+                    const int pc = c + 1, pr = r + 1;  // phi tile coords of (x, y)
+                    const float* row = sp[s % 3][pr];
+                    const int pcr = (x == w - 1) ? pc : pc + 1;
+                    const int prr = (y == 0) ? pr + 1 : (y == h - 1 ? pr : pr + 1);
+                    const int prl = (y == 0) ? pr : (y == h - 1 ? pr - 1 : pr - 1);
+                    g.x = (row[pcr] - row[pc - 1]) * 0.5f;
+                    g.y = (sp[s % 3][prr][pc] - sp[s % 3][prl][pc]) * 0.5f;
+                    g.z = (sp[zr % 3][pr][pc] - sp[zl % 3][pr][pc]) * 0.5f;
+                }
+                float r_;
+                if constexpr (LIT) {
+                    const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    snx[gb][r][c] = (float)((double)g.x / ((double)M + 1e-6));
+                    sny[gb][r][c] = (float)((double)g.y / ((double)M + 1e-6));
+                    r_ = 0.f;
+                    if (r >= 1 && r <= TY && c >= 1 && c <= TX) {
+                        snz[r - 1][c - 1] = (float)((double)g.z / ((double)M + 1e-6));
+                        sgz[r - 1][c - 1] = g.z;
+                    }
+                } else {
+                    const float M = __builtin_amdgcn_sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    r_ = __builtin_amdgcn_rcpf(M + 1e-6f);
+                    snx[gb][r][c] = g.x * r_;
+                    sny[gb][r][c] = g.y * r_;
+                    if (r >= 1 && r <= TY && c >= 1 && c <= TX) {
+                        snz[r - 1][c - 1] = g.z * r_;
+                        sgz[r - 1][c - 1] = g.z;
+                    }
+                }
+                sgx[gb][r][c] = g.x;
+                sgy[gb][r][c] = g.y;
+            }
+            __syncthreads();
+            nz_0 = snz[ty][tx];
+            gz_0 = sgz[ty][tx];
+            // ---- output plane s-1 (its D_z partners: plane s and plane s-2, or s-1 itself at z = 0)
+            const int z = s - 1;
+            if (z >= z0 && z < zend) {
+                if (z == 0) emit(z, nz_0, gz_0, nz_m1, gz_m1);
+                else emit(z, nz_0, gz_0, nz_m2, gz_m2);
+            }
+            // the last plane of the volume: D_z partners are itself and l-2
+            if (s == l - 1 && s >= z0 && s < zend) emit(s, nz_0, gz_0, nz_m1, gz_m1);
+            nz_m2 = nz_m1; gz_m2 = gz_m1;
+            nz_m1 = nz_0; gz_m1 = gz_0;
+            __syncthreads();
+        }
+    }
+}
+
 __global__ void transform_kernel(float* tr, const float* phi, int64_t n, int lit, float eps, float inv_eps) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         tr[i] = lit ? heav_literal(phi[i], eps) : heav_centred(phi[i], inv_eps);
 }
 
-template <int P, bool LIT, int TX, int TY, int NT>
+template <int P, bool LIT, int TX, int TY, int NT, int WPE = 0>
 Sweep1Variant s1v(const char* name) {
-    return Sweep1Variant{P, LIT ? 1 : 0, TX, TY, NT, (const void*)&sweep1_kernel<P, LIT, TX, TY, NT>, name};
+    return Sweep1Variant{P, LIT ? 1 : 0, TX, TY, NT, (const void*)&sweep1_kernel<P, LIT, TX, TY, NT, WPE>, name};
 }
 template <bool LIT, int TX, int TY>
 Sweep2Variant s2v(const char* name) {
-    return Sweep2Variant{LIT ? 1 : 0, TX, TY, (const void*)&sweep2_kernel<LIT, TX, TY>, name};
+    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 1, (const void*)&sweep2_kernel<LIT, TX, TY>, nullptr, name};
+}
+template <bool LIT, int TX, int TY, int WPE>
+Sweep2Variant s2l(const char* name) {
+    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 1, (const void*)&sweep2_lds_kernel<LIT, TX, TY, WPE>, nullptr, name};
+}
+template <bool LIT, int TX, int TY>
+Sweep2Variant s2s(const char* name) {
+    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 1, (const void*)&sweep2_inner_kernel<LIT, TX, TY>,
+                         (const void*)&sweep2_shell_kernel<LIT>, name};
 }
 
 }  // namespace
 
 const std::vector<Sweep1Variant>& sweep1_variants() {
     static const std::vector<Sweep1Variant> v = {
-        s1v<21, false, 32, 32, 512>("p21_fast_32x32"),
-        s1v<21, false, 64, 16, 512>("p21_fast_64x16"),
+        s1v<21, false, 32, 16, 512, 4>("p21_fast_32x16w4"),
         s1v<21, false, 32, 16, 512>("p21_fast_32x16"),
+        s1v<21, false, 16, 20, 320, 4>("p21_fast_16x20w4"),
+        s1v<21, false, 16, 20, 320, 5>("p21_fast_16x20w5"),
+        s1v<21, false, 32, 20, 640, 5>("p21_fast_32x20w5"),
+        s1v<21, false, 32, 32, 1024, 4>("p21_fast_32x32t1k"),
+        s1v<21, false, 32, 32, 512>("p21_fast_32x32"),
+        s1v<21, false, 32, 20, 320>("p21_fast_32x20"),
         s1v<21, true, 32, 16, 512>("p21_lit_32x16"),
         s1v<7, false, 32, 32, 512>("p7_fast_32x32"),
         s1v<7, true, 32, 16, 512>("p7_lit_32x16"),
@@ -280,8 +796,12 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
 
 const std::vector<Sweep2Variant>& sweep2_variants() {
     static const std::vector<Sweep2Variant> v = {
+        s2s<false, 64, 4>("fast_split_64x4"),
+        s2s<false, 64, 2>("fast_split_64x2"),
+        s2s<false, 32, 8>("fast_split_32x8"),
         s2v<false, 64, 4>("fast_64x4"),
-        s2v<false, 32, 8>("fast_32x8"),
+        s2l<false, 32, 16, 4>("fast_lds_32x16"),
+        s2s<true, 64, 4>("lit_split_64x4"),
         s2v<true, 64, 4>("lit_64x4"),
     };
     return v;
@@ -327,6 +847,7 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     tp.blocks = blocks;
     tp.tiles1.clear();
     tp.tiles2.clear();
+    tp.tiles3.clear();
     tp.total_voxels = 0;
     const int tx1 = V1[s1].tx, ty1 = V1[s1].ty, tx2 = V2[s2].tx, ty2 = V2[s2].ty;
     for (size_t b = 0; b < blocks.size(); ++b) {
@@ -335,9 +856,20 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
         for (int z = 0; z < bd.l; z += zc1)
             for (int y = 0; y < bd.h; y += ty1)
                 for (int x = 0; x < bd.w; x += tx1) tp.tiles1.push_back(Tile{(int)b, x, y, z});
-        for (int z = 0; z < bd.l; z += zc2)
-            for (int y = 0; y < bd.h; y += ty2)
-                for (int x = 0; x < bd.w; x += tx2) tp.tiles2.push_back(Tile{(int)b, x, y, z});
+        if (V2[s2].shell_fn) {
+            for (int z = 2; z < bd.l - 2; z += zc2)
+                for (int y = 2; y < bd.h - 2; y += ty2)
+                    for (int x = 2; x < bd.w - 2; x += tx2) tp.tiles2.push_back(Tile{(int)b, x, y, z});
+            const int cols = 4;
+            const int64_t shell = 4 * (int64_t)bd.w * bd.h + 4 * (int64_t)bd.w * (bd.l - 4) +
+                                  (int64_t)cols * (bd.h - 4) * (bd.l - 4);
+            for (int64_t k = 0; k < shell; k += 256)
+                tp.tiles3.push_back(Tile{(int)b, (int)k, (int)std::min<int64_t>(256, shell - k), cols});
+        } else {
+            for (int z = 0; z < bd.l; z += zc2)
+                for (int y = 0; y < bd.h; y += ty2)
+                    for (int x = 0; x < bd.w; x += tx2) tp.tiles2.push_back(Tile{(int)b, x, y, z});
+        }
     }
     xcd_order(tp.tiles1);
     xcd_order(tp.tiles2);
@@ -349,11 +881,16 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     };
     tp.grid1 = grid_of(tp.tiles1.size(), 2);
     tp.grid2 = grid_of(tp.tiles2.size(), 8);
+    tp.grid3 = tp.tiles3.empty() ? 0 : grid_of(tp.tiles3.size(), 8);
     RSF_TRY(tp.d_tiles1.ensure(tp.tiles1.size() * sizeof(Tile)));
     RSF_TRY(tp.d_tiles2.ensure(tp.tiles2.size() * sizeof(Tile)));
     RSF_TRY(tp.d_blocks.ensure(blocks.size() * sizeof(BlockDesc)));
     RSF_HIP(hipMemcpy(tp.d_tiles1.p, tp.tiles1.data(), tp.tiles1.size() * sizeof(Tile), hipMemcpyHostToDevice));
     RSF_HIP(hipMemcpy(tp.d_tiles2.p, tp.tiles2.data(), tp.tiles2.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    if (!tp.tiles3.empty()) {
+        RSF_TRY(tp.d_tiles3.ensure(tp.tiles3.size() * sizeof(Tile)));
+        RSF_HIP(hipMemcpy(tp.d_tiles3.p, tp.tiles3.data(), tp.tiles3.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    }
     RSF_HIP(hipMemcpy(tp.d_blocks.p, blocks.data(), blocks.size() * sizeof(BlockDesc), hipMemcpyHostToDevice));
     return RSF_OK;
 }
@@ -381,7 +918,14 @@ int launch_sweep2(const TilePlan& tp, const SweepArgs& a, const rsf_params* p, h
                     (void*)&tp.d_blocks.p, (void*)&tp.d_tiles2.p, (void*)&ntiles, (void*)&tp.zc2,
                     (void*)&p->dt,  (void*)&p->nu,   (void*)&p->mu,     (void*)&p->epsilon,
                     (void*)&inv_eps};
-    RSF_HIP(hipLaunchKernel(v.fn, dim3(tp.grid2), dim3(v.tx * v.ty), args, 0, s));
+    if (ntiles > 0) RSF_HIP(hipLaunchKernel(v.fn, dim3(tp.grid2), dim3(v.tx * v.ty), args, 0, s));
+    if (v.shell_fn && tp.grid3 > 0) {
+        const int nitems = (int)tp.tiles3.size();
+        void* sargs[] = {(void*)&a.phi,  (void*)&a.F,      (void*)&a.phi_out, (void*)&a.tr_out,
+                         (void*)&tp.d_blocks.p, (void*)&tp.d_tiles3.p, (void*)&nitems, (void*)&p->dt,
+                         (void*)&p->nu, (void*)&p->mu, (void*)&p->epsilon, (void*)&inv_eps};
+        RSF_HIP(hipLaunchKernel(v.shell_fn, dim3(tp.grid3), dim3(256), sargs, 0, s));
+    }
     return RSF_OK;
 }
 
