@@ -178,11 +178,13 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (scale for rocprofv3 byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the RSF engine runs each volume or batch of blocks on a single "
-              "device; multi-GPU sharding is a design note only (see DESIGN.md)", file=sys.stderr)
+              "device; sharding over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -192,6 +194,15 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration > 0:
+        n = args.copy_calibration << 18
+        a = torch.arange(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 5, "ok": bool(torch.equal(a, b))}))
+        return
     rsf_amd.lib()
     mode = rsf_amd.MODE_FAST if args.mode == "fast" else rsf_amd.MODE_LITERAL
     ctx = rsf_amd.Context(0)

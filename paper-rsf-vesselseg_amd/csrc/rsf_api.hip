@@ -241,9 +241,11 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
         while (zc > zmin && cols * ((lmax + zc - 1) / zc) < (int64_t)per_cu * c->num_cus) zc = (zc + 1) / 2;
         return std::max(zc, 1);
     };
-    *zc1 = chunk(sweep1_variants()[*s1].tx, sweep1_variants()[*s1].ty, 2, 4 * d.P);
-    *zc2 = std::min(chunk(sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, sweep2_variants()[*s2].ty, 4, 16),
-                    64);
+    // no compiled variant for this P (or mode): the caller takes the unfused path
+    *zc1 = *s1 >= 0 ? chunk(sweep1_variants()[*s1].tx, sweep1_variants()[*s1].ty, 2, 4 * d.P) : std::min(lmax, 128);
+    *zc2 = *s2 >= 0 ? std::min(chunk(sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, sweep2_variants()[*s2].ty,
+                                     4, 16), 64)
+                    : std::min(lmax, 64);
     // explicit overrides: RSF_SWEEP1 / RSF_SWEEP2 = "<variant name>:<z chunk>"
     for (int which = 1; which <= 2; ++which) {
         const char* env = std::getenv(which == 1 ? "RSF_SWEEP1" : "RSF_SWEEP2");
