@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <initializer_list>
 #include <list>
 #include <map>
 #include <memory>
@@ -228,7 +229,7 @@ int max_dim_l(const std::vector<BlockDesc>& b) {
 
 // Chooses kernel variants and z-chunks for a problem: tuned if known, else defaults.
 void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const Derived& d, int literal,
-                   int* s1, int* s2, int* zc1, int* zc2) {
+                   bool aligned, int* s1, int* s2, int* zc1, int* zc2) {
     *s1 = default_sweep1(d.P, literal);
     *s2 = default_sweep2(literal);
     const int lmax = max_dim_l(blocks);
@@ -268,19 +269,36 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
             }
         }
     }
-    if (std::getenv("RSF_SWEEP1") || std::getenv("RSF_SWEEP2")) return;
-    auto it = c->tune.find(tune_key(blocks, d.P, literal));
-    if (it != c->tune.end()) {
-        const int a = find_variant1(it->second.s1), b = find_variant2(it->second.s2);
-        if (a >= 0 && sweep1_variants()[a].P == d.P && sweep1_variants()[a].literal == literal) {
-            *s1 = a;
-            *zc1 = it->second.zc1;
-        }
-        if (b >= 0 && sweep2_variants()[b].literal == literal) {
-            *s2 = b;
-            *zc2 = it->second.zc2;
+    if (!std::getenv("RSF_SWEEP1") && !std::getenv("RSF_SWEEP2")) {
+        auto it = c->tune.find(tune_key(blocks, d.P, literal));
+        if (it != c->tune.end()) {
+            const int a = find_variant1(it->second.s1), b = find_variant2(it->second.s2);
+            if (a >= 0 && sweep1_variants()[a].P == d.P && sweep1_variants()[a].literal == literal) {
+                *s1 = a;
+                *zc1 = it->second.zc1;
+            }
+            if (b >= 0 && sweep2_variants()[b].literal == literal) {
+                *s2 = b;
+                *zc2 = it->second.zc2;
+            }
         }
     }
+    // The split and vector sweep-2 kernels address a block through 32-bit buffer
+    // offsets; blocks of 2^30 voxels and more take the direct kernel.
+    bool small = true;
+    for (const auto& b : blocks)
+        small = small && 4.0 * ((double)b.w * b.h * b.l + 4.0 * b.w * b.h + 4.0 * b.w + 64.0) < 4294967295.0;
+    if (!small && *s2 >= 0 && sweep2_variants()[*s2].shell_fn) *s2 = find_variant2(literal ? "lit_64x4" : "fast_64x4");
+    // the vector interior reads 16-byte groups: caller buffers must start on a 16-byte boundary
+    if (!aligned && *s2 >= 0 && sweep2_variants()[*s2].vec == 4)
+        *s2 = find_variant2(literal ? "lit_split_64x4" : "fast_split_64x4");
+}
+
+// true when every pointer is 16-byte aligned (null pointers are ignored)
+bool aligned16(std::initializer_list<const void*> ptrs) {
+    for (const void* q : ptrs)
+        if (q && ((uintptr_t)q & 15) != 0) return false;
+    return true;
 }
 
 Runner* get_runner(rsf_context* c, const std::vector<BlockDesc>& blocks, int s1, int s2, int zc1, int zc2,
@@ -341,13 +359,13 @@ struct Prepared {
     bool fused = true;
 };
 
-int prepare(rsf_context* c, const rsf_block* blocks, int nblocks, const rsf_params* p, Prepared* pr) {
+int prepare(rsf_context* c, const rsf_block* blocks, int nblocks, const rsf_params* p, bool aligned, Prepared* pr) {
     RSF_TRY(derive(p, &pr->d));
     RSF_TRY(validate_blocks(blocks, nblocks, pr->d, &pr->blocks, &pr->extent));
     pr->literal = p->mode == RSF_MODE_LITERAL;
     pr->sw = (int)p->sigma2;
     int s1, s2, zc1, zc2;
-    pick_variants(c, pr->blocks, pr->d, pr->literal, &s1, &s2, &zc1, &zc2);
+    pick_variants(c, pr->blocks, pr->d, pr->literal, aligned, &s1, &s2, &zc1, &zc2);
     pr->fused = s1 >= 0 && s2 >= 0;
     if (!pr->fused) return RSF_OK;
     int st = RSF_OK;
@@ -654,7 +672,7 @@ int rsf_step_blocks_device(rsf_context* c, const rsf_block* blocks, int nblocks,
     RSF_TRY(check_dtype(image_dtype));
     if (!image || !phi_in || !phi_out) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
     Prepared pr;
-    RSF_TRY(prepare(c, blocks, nblocks, p, &pr));
+    RSF_TRY(prepare(c, blocks, nblocks, p, aligned16({phi_in, phi_out}), &pr));
     const int u8 = image_dtype == RSF_DTYPE_U8;
     if (!pr.fused) {
         if (nblocks != 1) return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel", pr.d.P);
@@ -680,7 +698,7 @@ int rsf_run_device(rsf_context* c, const void* image, int image_dtype, float* ph
     if (!image || !phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
     rsf_block b{0, w, h, l};
     Prepared pr;
-    RSF_TRY(prepare(c, &b, 1, p, &pr));
+    RSF_TRY(prepare(c, &b, 1, p, aligned16({phi}), &pr));
     const int u8 = image_dtype == RSF_DTYPE_U8;
     if (!pr.fused) {
         // unfused pipeline for tap counts without a fused specialisation
@@ -731,7 +749,7 @@ int rsf_run_batched_device(rsf_context* c, const rsf_block* blocks, int nblocks,
     RSF_TRY(check_dtype(image_dtype));
     if (!image || !phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
     Prepared pr;
-    RSF_TRY(prepare(c, blocks, nblocks, p, &pr));
+    RSF_TRY(prepare(c, blocks, nblocks, p, aligned16({phi}), &pr));
     if (!pr.fused) return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel for batched runs", pr.d.P);
     return run_prepared(c, pr, image, image_dtype == RSF_DTYPE_U8, phi, p, nullptr);
 }
@@ -742,7 +760,7 @@ int rsf_profile_sweeps(rsf_context* c, const rsf_block* blocks, int nblocks, con
     RSF_TRY(check_dtype(image_dtype));
     if (!image || !phi || !ms_sweep1 || !ms_sweep2) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL argument");
     Prepared pr;
-    RSF_TRY(prepare(c, blocks, nblocks, p, &pr));
+    RSF_TRY(prepare(c, blocks, nblocks, p, aligned16({phi}), &pr));
     if (!pr.fused || pr.sw != 0) return set_error(RSF_ERR_UNSUPPORTED, "profiling covers the fused sigma2 = 0 path");
     const int u8 = image_dtype == RSF_DTYPE_U8;
     RSF_TRY(ensure_workspace(c, pr));
@@ -916,7 +934,7 @@ int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks
         return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
     if (grad_in == grad_out || grad_in == phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "grad_in must not alias");
     Prepared pr;
-    RSF_TRY(prepare(c, blocks, nblocks, p, &pr));
+    RSF_TRY(prepare(c, blocks, nblocks, p, aligned16({phi}), &pr));
     if (!pr.fused) return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel", pr.d.P);
     if (pr.sw != 0) return set_error(RSF_ERR_UNSUPPORTED, "the device adjoint covers sigma2 = 0");
     const int u8 = image_dtype == RSF_DTYPE_U8;

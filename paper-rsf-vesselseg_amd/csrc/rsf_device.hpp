@@ -46,7 +46,29 @@ __device__ __forceinline__ float heav_centred(float phi, float inv_eps) {
     return atanf(phi * inv_eps) * (float)(1.0 / kPi);
 }
 __device__ __forceinline__ float dirac_fast(float phi, float eps) {
-    return (float)(1.0 / kPi) * (eps * __builtin_amdgcn_rcpf((eps * eps) + (phi * phi)));
+    return (float)(1.0 / kPi) * (eps * __builtin_amdgcn_rcpf(__builtin_fmaf(phi, phi, eps * eps)));
+}
+
+// The sweep-2 kernels write their fused multiply-adds out explicitly (and
+// contract nothing else in these helpers), so that every kernel variant
+// rounds the update identically whichever of them a voxel runs on.
+// (a + b + c)/2 in the reference's order ((a/2 + b/2) + c/2); the halvings are
+// exact, so the two fused forms round exactly like the separate operations.
+__device__ __forceinline__ float half_sum3(float a, float b, float c) {
+    return __builtin_fmaf(c, 0.5f, __builtin_fmaf(a, 0.5f, b * 0.5f));
+}
+// |g|^2 = gx^2 + gy^2 + gz^2 (kernel.cu:914-933).
+__device__ __forceinline__ float mag2(float gx, float gy, float gz) {
+    return __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
+}
+// phi' = ((phi - (delta*F)*dt) + ((nu*dt)*delta)*kappa) + mu*(dt*(L - kappa))
+// (kernel.cu:384-386, F = lambda1*Eo - lambda2*Ei); nudt = nu*dt.
+__device__ __forceinline__ float phi_update(float p0, float d, float F, float kap, float lap, float dt, float nudt,
+                                            float mu) {
+#pragma clang fp contract(off)
+    const float a = __builtin_fmaf(-(d * F), dt, p0);
+    const float b = __builtin_fmaf(nudt * d, kap, a);
+    return __builtin_fmaf(mu, dt * (lap - kap), b);
 }
 
 // ------------------------------------------------- difference operator rules

@@ -112,6 +112,7 @@ struct Sweep2Variant {
     int vec;               // consecutive x voxels per thread of the interior kernel
     const void* fn;        // whole-block kernel, or the interior kernel when shell_fn is set
     const void* shell_fn;  // boundary-shell kernel of a split variant (else null)
+    const void* scalar_fn; // vector variants: interior kernel of blocks with unaligned rows
     const char* name;
 };
 const std::vector<Sweep1Variant>& sweep1_variants();
@@ -121,9 +122,10 @@ struct TilePlan {
     int s1 = -1, s2 = -1;        // variant indices
     int zc1 = 0, zc2 = 0;        // z-chunk of each sweep
     int grid1 = 0, grid2 = 0;    // persistent grid sizes
-    int grid3 = 0;
-    std::vector<Tile> tiles1, tiles2, tiles3;  // sweep 1, sweep 2 (interior), sweep 2 shell
-    DevBuf d_tiles1, d_tiles2, d_tiles3, d_blocks;
+    int grid3 = 0, grid4 = 0;
+    // sweep 1; sweep 2 interior; sweep 2 shell; sweep 2 scalar interior of unaligned blocks
+    std::vector<Tile> tiles1, tiles2, tiles3, tiles4;
+    DevBuf d_tiles1, d_tiles2, d_tiles3, d_tiles4, d_blocks;
     std::vector<BlockDesc> blocks;
     int64_t total_voxels = 0;
 };

@@ -17,6 +17,7 @@
 // Both sweeps walk a tile list (persistent workgroups), so the same kernels
 // serve one volume or many uneven blocks.
 #include <algorithm>
+#include <type_traits>
 #include <cstdio>
 
 #include "rsf_internal.hpp"
@@ -78,7 +79,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WP
         const int zend = min(z0 + zc, l);
         const int nplanes = (zend - z0) + P - 1;
         // float4 path: rows 16-byte aligned and the aligned window inside the row
-        const bool vec = ((off | (int64_t)w) & 3) == 0 && x0 - AL >= 0 && x0 + TX + AL <= w;
+        const bool vec = ((off | (int64_t)w) & 3) == 0 && x0 - AL >= 0 && x0 + TX + AL <= w &&
+                         ((uintptr_t)tr & 15) == 0 &&
+                         (img_u8 ? ((uintptr_t)img8 & 3) == 0 : ((uintptr_t)imgf & 15) == 0);
 
         // prefetch registers: tr and image of this thread's float4 groups
         float4 pt[PF], pi[PF];
@@ -278,6 +281,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WP
 }
 #undef RSF_ZCASE
 
+// Sweep 2 kernels: no implicit contraction.  Each kernel variant (and each
+// code path inside one, e.g. a chunk's first plane) then rounds every voxel's
+// update the same way; the fused multiply-adds they want are written out
+// (mag2, half_sum3, phi_update, dirac_fast).
+#pragma clang fp contract(off)
+
 // One voxel of sweep 2 with the full reference edge rules (boundary shell).
 template <bool LIT>
 __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const float* __restrict__ F,
@@ -301,7 +310,7 @@ __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const 
     float nRx, nLx, nRy, nLy, nRz, nLz;
     if constexpr (LIT) {
         auto nrm = [](const Grad& g, float c) {
-            const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+            const float M = sqrtf(mag2(g.x, g.y, g.z));
             return (float)((double)c / ((double)M + 1e-6));
         };
         nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -309,38 +318,40 @@ __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const 
         nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
     } else {
         auto nrm = [](const Grad& g, float c) {
-            const float M = __builtin_amdgcn_sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+            const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
             return c * __builtin_amdgcn_rcpf(M + 1e-6f);
         };
         nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
         nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
         nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
     }
-    const float kap = ((nRx - nLx) * 0.5f + (nRy - nLy) * 0.5f) + (nRz - nLz) * 0.5f;
-    const float lap = ((gRx.x - gLx.x) * 0.5f + (gRy.y - gLy.y) * 0.5f) + (gRz.z - gLz.z) * 0.5f;
+    const float kap = half_sum3(nRx - nLx, nRy - nLy, nRz - nLz);
+    const float lap = half_sum3(gRx.x - gLx.x, gRy.y - gLy.y, gRz.z - gLz.z);
     const float p0 = f[i];
     const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
-    const float p1 = ((p0 - (d * F[i]) * dt) + ((nu * dt) * d) * kap) + mu * (dt * (lap - kap));
+    const float p1 = phi_update(p0, d, F[i], kap, lap, dt, nu * dt, mu);
     phi_out[i] = p1;
     tr_out[i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
 }
 
-// Sweep 2, boundary shell: voxels within 2 of a face of their block (and,
-// next to the vector interior kernel, the columns x = 2, 3, w-4, w-3).  Work
-// items (block, first shell index, count, columns) enumerate, per block, the 4
-// planes z in {0,1,l-2,l-1}, then the rows y in {0,1,h-2,h-1} of the inner
-// planes, then `cols` columns at the two x ends of the inner rows.
+// Sweep 2, boundary shell: voxels within 2 of a face of their block, and,
+// for blocks whose interior runs on the vector kernel, the columns that kernel
+// leaves out (x = 2, 3 and x >= 4 + 4*((w-8)/4)).  Work items (block, first
+// shell index, count, vector flag) enumerate, per block, the 4 planes z in
+// {0,1,l-2,l-1}, then the rows y in {0,1,h-2,h-1} of the inner planes, then the
+// boundary columns of the inner rows.
 template <bool LIT>
 __global__ __launch_bounds__(256) void sweep2_shell_kernel(
     const float* __restrict__ phi_all, const float* __restrict__ F, float* __restrict__ phi_out,
     float* __restrict__ tr_out, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ items,
     int nitems, float dt, float nu, float mu, float eps, float inv_eps) {
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const int cols = items[it].z0;  // boundary columns per inner row: 4 or 8
         const Tile t = items[it];
         if (t.b < 0 || (int)threadIdx.x >= t.y0) continue;
         const BlockDesc bd = blocks[t.b];
         const int w = bd.w, h = bd.h, l = bd.l;
+        // boundary columns of the inner rows: x < cl and x >= w - cr
+        const int cl = t.z0 ? 4 : 2, cr = t.z0 ? w - 4 - 4 * ((w - 8) >> 2) : 2, cols = cl + cr;
         int64_t k = (int64_t)t.x0 + threadIdx.x;
         const int64_t cA = 4 * (int64_t)w * h, cB = 4 * (int64_t)w * (l - 4);
         int x, y, z;
@@ -363,8 +374,8 @@ __global__ __launch_bounds__(256) void sweep2_shell_kernel(
             z = 2 + (int)(k / per);
             const int r = (int)(k - (int64_t)(z - 2) * per);
             y = 2 + r / cols;
-            const int xx = r % cols, half = cols / 2;
-            x = xx < half ? xx : w - cols + xx;
+            const int xx = r % cols;
+            x = xx < cl ? xx : w - cols + xx;
         }
         sweep2_voxel<LIT>(phi_all + bd.off, F + bd.off, phi_out + bd.off, tr_out + bd.off, x, y, z, w, h, l, dt, nu,
                           mu, eps, inv_eps);
@@ -433,7 +444,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
             float nRx, nLx, nRy, nLy, nRz, nLz;
             if constexpr (LIT) {
                 auto nrm = [](const Grad& g, float cc) {
-                    const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    const float M = sqrtf(mag2(g.x, g.y, g.z));
                     return (float)((double)cc / ((double)M + 1e-6));
                 };
                 nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -441,17 +452,17 @@ __global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
                 nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
             } else {
                 auto nrm = [](const Grad& g, float cc) {
-                    const float M = __builtin_amdgcn_sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
                     return cc * __builtin_amdgcn_rcpf(M + 1e-6f);
                 };
                 nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
                 nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
                 nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
             }
-            const float kap = ((nRx - nLx) * 0.5f + (nRy - nLy) * 0.5f) + (nRz - nLz) * 0.5f;
-            const float lap = ((gRx.x - gLx.x) * 0.5f + (gRy.y - gLy.y) * 0.5f) + (gRz.z - gLz.z) * 0.5f;
+            const float kap = half_sum3(nRx - nLx, nRy - nLy, nRz - nLz);
+            const float lap = half_sum3(gRx.x - gLx.x, gRy.y - gLy.y, gRz.z - gLz.z);
             const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
-            const float p1 = ((c0 - (d * r[19]) * dt) + ((nu * dt) * d) * kap) + mu * (dt * (lap - kap));
+            const float p1 = phi_update(c0, d, r[19], kap, lap, dt, nu * dt, mu);
             const float t1 = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, p1), rO, (int)v, 0, 0);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, t1), rT, (int)v, 0, 0);
@@ -471,6 +482,204 @@ __global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
             compute(B, v);
             ++z;
             v += vstep;
+        }
+    }
+}
+
+// Sweep 2, vector interior.  Each thread owns 4 consecutive x voxels
+// (x0..x0+3, x0 = 4 + 4q) of one row y and marches them through z.  Rows are
+// read with 8- and 16-byte buffer loads (rows of a block must start on a
+// 16-byte boundary: offset and w multiples of 4), and a 4-deep register ring
+// along z keeps
+//   C: phi(y, z-1 .. z+2) at x0-2 .. x0+5,
+//   U, D: phi(y+1, .), phi(y-1, .) at x0-1 .. x0+4 for z-1 .. z+1,
+//   the gradient at (x0..x0+3, y, z') computed once, at step z'-1 (as the
+//   z+1 neighbour), and used again at z' (as x neighbours) and z'+1 (as the
+//   z-1 neighbour).
+// Per step a thread loads one new C row, one new U and D row, the rows
+// y+-2 and F (12 loads for 4 voxels) and evaluates 14 gradients (3.5 per
+// voxel).  Interior rules only: x in [4, 4+4*nq), y and z in [2, n-2).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template <bool LIT>
+__device__ __forceinline__ float norm_comp(float c, float gx, float gy, float gz) {
+    if constexpr (LIT) {
+        const float M = sqrtf(mag2(gx, gy, gz));
+        return (float)((double)c / ((double)M + 1e-6));
+    } else {
+        const float M = __builtin_amdgcn_sqrtf(mag2(gx, gy, gz));
+        return c * __builtin_amdgcn_rcpf(M + 1e-6f);
+    }
+}
+
+template <bool LIT, int TX, int TY, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep2_vec_kernel(
+    const float* __restrict__ phi_all, const float* __restrict__ F, float* __restrict__ phi_out,
+    float* __restrict__ tr_out, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
+    int ntiles, int zc, float dt, float nu, float mu, float eps, float inv_eps) {
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
+        const Tile tl = tiles[item];
+        if (tl.b < 0) continue;
+        const BlockDesc bd = blocks[tl.b];
+        const int w = bd.w, h = bd.h, l = bd.l;
+        const int xend = 4 + 4 * ((w - 8) >> 2);
+        const int x0 = tl.x0 + 4 * tx, y = tl.y0 + ty;
+        if (x0 >= xend || y >= h - 2) continue;
+        const int zb = tl.z0, zend = min(tl.z0 + zc, l - 2);
+        const int W = w, P = w * h;
+        const int shift = 2 * P + 2 * W + 4;
+        const float* f = phi_all + bd.off;
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(f - shift), 0, (int)0xffffffff, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rF =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(F + bd.off), 0, (int)0xffffffff, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rO =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(phi_out + bd.off), 0, (int)0xffffffff, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rT =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(tr_out + bd.off), 0, (int)0xffffffff, 0x00020000);
+        // byte offset of row (y+dy, z+dz) minus 4 elements: voffset + 0..32 bytes then
+        // addresses x0-4 .. x0+4, and every sum stays non-negative
+        auto S = [&](int dy, int dz) { return (int)(4u * (uint32_t)(shift + dz * P + dy * W - 4)); };
+        const int sC = S(0, 0), sU = S(1, 0), sD = S(-1, 0), sU2 = S(2, 0), sD2 = S(-2, 0);
+        const uint32_t vstep = 4u * (uint32_t)P;
+        auto u2f = [](uint32_t u) { return __builtin_bit_cast(float, u); };
+        // centre-type row (x0-2 .. x0+5) at voxel byte offset v + row (dy, dz)
+        auto load8 = [&](float (&r)[8], uint32_t v, int so) {
+            const u32x2 a = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(v + 8), so, 0);
+            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(v + 16), so, 0);
+            const u32x2 c = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(v + 32), so, 0);
+            r[0] = u2f(a[0]); r[1] = u2f(a[1]);
+            r[2] = u2f(b[0]); r[3] = u2f(b[1]); r[4] = u2f(b[2]); r[5] = u2f(b[3]);
+            r[6] = u2f(c[0]); r[7] = u2f(c[1]);
+        };
+        // neighbour-type row (x0-1 .. x0+4)
+        auto load6 = [&](float (&r)[6], uint32_t v, int so) {
+            r[0] = u2f(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(v + 12), so, 0));
+            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(v + 16), so, 0);
+            r[1] = u2f(b[0]); r[2] = u2f(b[1]); r[3] = u2f(b[2]); r[4] = u2f(b[3]);
+            r[5] = u2f(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(v + 32), so, 0));
+        };
+        auto load4 = [&](float (&r)[4], uint32_t v, int so) {
+            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(v + 16), so, 0);
+            r[0] = u2f(b[0]); r[1] = u2f(b[1]); r[2] = u2f(b[2]); r[3] = u2f(b[3]);
+        };
+        auto loadF = [&](float (&r)[4], uint32_t v) {
+            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rF, (int)v, 0, 0);
+            r[0] = u2f(b[0]); r[1] = u2f(b[1]); r[2] = u2f(b[2]); r[3] = u2f(b[3]);
+        };
+        float C[4][8], U[4][6], D[4][6];          // phi rows, ring slot = (z' - zb) & 3
+        float Q2u[2][4], Q2d[2][4], FF[2][4];     // rows y+-2 and F of the current / next plane
+        float NZ[4][4], GZ[4][4], NX[4][4], GX[4][4];  // gradient at (x0+i, y, z')
+        const uint32_t v0 = 4u * (uint32_t)(zb * P + y * W + x0);
+        // the gradient at (x0+i, y, z') from the rows of planes z'-1, z', z'+1
+        auto zpoint = [&](int i, const float (&cm)[8], const float (&c0)[8], const float (&cp)[8],
+                          const float (&u0)[6], const float (&d0)[6], float& nx, float& gx, float& nz, float& gz) {
+            gx = (c0[i + 3] - c0[i + 1]) * 0.5f;
+            const float gy = (u0[i + 1] - d0[i + 1]) * 0.5f;
+            gz = (cp[i + 2] - cm[i + 2]) * 0.5f;
+            nx = norm_comp<LIT>(gx, gx, gy, gz);
+            nz = norm_comp<LIT>(gz, gx, gy, gz);
+        };
+        // ---- prologue: rows of planes zb-2 .. zb+2, gradients at zb-1 and zb
+        {
+            float Cmm[8];
+            load8(Cmm, v0, S(0, -2));
+            load8(C[3], v0, S(0, -1));
+            load8(C[0], v0, sC);
+            load8(C[1], v0, S(0, 1));
+            load8(C[2], v0, S(0, 2));
+            load6(U[3], v0, S(1, -1));
+            load6(D[3], v0, S(-1, -1));
+            load6(U[0], v0, sU);
+            load6(D[0], v0, sD);
+            load6(U[1], v0, S(1, 1));
+            load6(D[1], v0, S(-1, 1));
+            load4(Q2u[0], v0, sU2);
+            load4(Q2d[0], v0, sD2);
+            loadF(FF[0], v0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float nx, gx;
+                zpoint(i, Cmm, C[3], C[0], U[3], D[3], nx, gx, NZ[3][i], GZ[3][i]);
+                zpoint(i, C[3], C[0], C[1], U[0], D[0], NX[0][i], GX[0][i], NZ[0][i], GZ[0][i]);
+            }
+        }
+        auto step = [&](auto S_, uint32_t v, int z) __attribute__((always_inline)) {
+            constexpr int s = decltype(S_)::value;
+            constexpr int km1 = (s + 3) & 3, k0 = s & 3, kp1 = (s + 1) & 3, kp2 = (s + 2) & 3;
+            constexpr int q = s & 1, qn = (s + 1) & 1;
+            float (&Cm)[8] = C[km1];
+            float (&C0)[8] = C[k0];
+            float (&Cp)[8] = C[kp1];
+            float (&Cpp)[8] = C[kp2];
+            // the two values of plane z-1 still needed, then that slot takes plane z+3
+            const float cmL = Cm[1], cmR = Cm[6];
+            if (z + 1 < zend) {
+                load8(C[km1], v + vstep, S(0, 2));
+                load6(U[kp2], v + vstep, S(1, 1));
+                load6(D[kp2], v + vstep, S(-1, 1));
+                load4(Q2u[qn], v + vstep, sU2);
+                load4(Q2d[qn], v + vstep, sD2);
+                loadF(FF[qn], v + vstep);
+            }
+            // gradients at (x0+i, y, z+1): the z+1 neighbours now, x neighbours next step
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                zpoint(i, C0, Cp, Cpp, U[kp1], D[kp1], NX[kp1][i], GX[kp1][i], NZ[kp1][i], GZ[kp1][i]);
+            // x neighbours at the quad's ends: (x0-1, y, z) and (x0+4, y, z)
+            float nxL, gxL, nxR, gxR;
+            {
+                gxL = (C0[2] - C0[0]) * 0.5f;
+                const float gy = (U[k0][0] - D[k0][0]) * 0.5f, gz = (Cp[1] - cmL) * 0.5f;
+                nxL = norm_comp<LIT>(gxL, gxL, gy, gz);
+            }
+            {
+                gxR = (C0[7] - C0[5]) * 0.5f;
+                const float gy = (U[k0][5] - D[k0][5]) * 0.5f, gz = (Cp[6] - cmR) * 0.5f;
+                nxR = norm_comp<LIT>(gxR, gxR, gy, gz);
+            }
+            const float nxs[6] = {nxL, NX[k0][0], NX[k0][1], NX[k0][2], NX[k0][3], nxR};
+            const float gxs[6] = {gxL, GX[k0][0], GX[k0][1], GX[k0][2], GX[k0][3], gxR};
+            float p1[4], t1[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                // y neighbours (x0+i, y+-1, z)
+                const float gxu = (U[k0][i + 2] - U[k0][i]) * 0.5f;
+                const float gyu = (Q2u[q][i] - C0[i + 2]) * 0.5f;
+                const float gzu = (U[kp1][i + 1] - U[km1][i + 1]) * 0.5f;
+                const float nyu = norm_comp<LIT>(gyu, gxu, gyu, gzu);
+                const float gxd = (D[k0][i + 2] - D[k0][i]) * 0.5f;
+                const float gyd = (C0[i + 2] - Q2d[q][i]) * 0.5f;
+                const float gzd = (D[kp1][i + 1] - D[km1][i + 1]) * 0.5f;
+                const float nyd = norm_comp<LIT>(gyd, gxd, gyd, gzd);
+                const float kap = half_sum3(nxs[i + 2] - nxs[i], nyu - nyd, NZ[kp1][i] - NZ[km1][i]);
+                const float lap = half_sum3(gxs[i + 2] - gxs[i], gyu - gyd, GZ[kp1][i] - GZ[km1][i]);
+                const float c0 = C0[i + 2];
+                const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
+                p1[i] = phi_update(c0, d, FF[q][i], kap, lap, dt, nu * dt, mu);
+                t1[i] = LIT ? heav_literal(p1[i], eps) : heav_centred(p1[i], inv_eps);
+            }
+            u32x4 po, pt;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                po[i] = __builtin_bit_cast(uint32_t, p1[i]);
+                pt[i] = __builtin_bit_cast(uint32_t, t1[i]);
+            }
+            __builtin_amdgcn_raw_buffer_store_b128(po, rO, (int)v, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(pt, rT, (int)v, 0, 0);
+        };
+        uint32_t v = v0;
+        for (int z = zb; z < zend; z += 4) {
+            step(std::integral_constant<int, 0>{}, v, z);
+            if (z + 1 >= zend) break;
+            step(std::integral_constant<int, 1>{}, v + vstep, z + 1);
+            if (z + 2 >= zend) break;
+            step(std::integral_constant<int, 2>{}, v + 2 * vstep, z + 2);
+            if (z + 3 >= zend) break;
+            step(std::integral_constant<int, 3>{}, v + 3 * vstep, z + 3);
+            v += 4 * vstep;
         }
     }
 }
@@ -538,7 +747,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
                 float nRx, nLx, nRy, nLy, nRz, nLz;
                 if constexpr (LIT) {
                     auto nrm = [](const Grad& g, float cc) {
-                        const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                        const float M = sqrtf(mag2(g.x, g.y, g.z));
                         return (float)((double)cc / ((double)M + 1e-6));
                     };
                     nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -546,18 +755,18 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
                     nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
                 } else {
                     auto nrm = [](const Grad& g, float cc) {
-                        const float M = __builtin_amdgcn_sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                        const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
                         return cc * __builtin_amdgcn_rcpf(M + 1e-6f);
                     };
                     nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
                     nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
                     nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
                 }
-                const float kap = ((nRx - nLx) * 0.5f + (nRy - nLy) * 0.5f) + (nRz - nLz) * 0.5f;
-                const float lap = ((gRx.x - gLx.x) * 0.5f + (gRy.y - gLy.y) * 0.5f) + (gRz.z - gLz.z) * 0.5f;
+                const float kap = half_sum3(nRx - nLx, nRy - nLy, nRz - nLz);
+                const float lap = half_sum3(gRx.x - gLx.x, gRy.y - gLy.y, gRz.z - gLz.z);
                 const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
                 const float Fv = F[bd.off + i];
-                const float p1 = ((c0 - (d * Fv) * dt) + ((nu * dt) * d) * kap) + mu * (dt * (lap - kap));
+                const float p1 = phi_update(c0, d, Fv, kap, lap, dt, nu * dt, mu);
                 phi_out[bd.off + i] = p1;
                 tr_out[bd.off + i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
                 continue;
@@ -575,7 +784,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
             float nRx, nLx, nRy, nLy, nRz, nLz;
             if constexpr (LIT) {
                 auto nrm = [](const Grad& g, float c) {
-                    const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    const float M = sqrtf(mag2(g.x, g.y, g.z));
                     return (float)((double)c / ((double)M + 1e-6));
                 };
                 nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -583,19 +792,19 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
                 nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
             } else {
                 auto nrm = [](const Grad& g, float c) {
-                    const float M = __builtin_amdgcn_sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
                     return c * __builtin_amdgcn_rcpf(M + 1e-6f);
                 };
                 nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
                 nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
                 nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
             }
-            const float kap = ((nRx - nLx) * 0.5f + (nRy - nLy) * 0.5f) + (nRz - nLz) * 0.5f;
-            const float lap = ((gRx.x - gLx.x) * 0.5f + (gRy.y - gLy.y) * 0.5f) + (gRz.z - gLz.z) * 0.5f;
+            const float kap = half_sum3(nRx - nLx, nRy - nLy, nRz - nLz);
+            const float lap = half_sum3(gRx.x - gLx.x, gRy.y - gLy.y, gRz.z - gLz.z);
             const float p0 = f[i];
             const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
             const float Fv = F[bd.off + i];
-            const float p1 = ((p0 - (d * Fv) * dt) + ((nu * dt) * d) * kap) + mu * (dt * (lap - kap));
+            const float p1 = phi_update(p0, d, Fv, kap, lap, dt, nu * dt, mu);
             phi_out[bd.off + i] = p1;
             tr_out[bd.off + i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
         }
@@ -664,14 +873,12 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
             const int cxr = (x == w - 1) ? cx : cx + 1, cxl = cx - 1;
             const int cyr = (y == 0) ? cy + 1 : (y == h - 1 ? cy : cy + 1);
             const int cyl = (y == 0) ? cy : (y == h - 1 ? cy - 1 : cy - 1);
-            const float kap = ((snx[b][cy][cxr] - snx[b][cy][cxl]) * 0.5f + (sny[b][cyr][cx] - sny[b][cyl][cx]) * 0.5f) +
-                              (nzR - nzL) * 0.5f;
-            const float lap = ((sgx[b][cy][cxr] - sgx[b][cy][cxl]) * 0.5f + (sgy[b][cyr][cx] - sgy[b][cyl][cx]) * 0.5f) +
-                              (gzR - gzL) * 0.5f;
+            const float kap = half_sum3(snx[b][cy][cxr] - snx[b][cy][cxl], sny[b][cyr][cx] - sny[b][cyl][cx], nzR - nzL);
+            const float lap = half_sum3(sgx[b][cy][cxr] - sgx[b][cy][cxl], sgy[b][cyr][cx] - sgy[b][cyl][cx], gzR - gzL);
             const float p0 = sp[z % 3][ty + 2][tx + 2];
             const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
             const int64_t i = bd.off + (int64_t)z * plane + (int64_t)y * w + x;
-            const float p1 = ((p0 - (d * F[i]) * dt) + ((nu * dt) * d) * kap) + mu * (dt * (lap - kap));
+            const float p1 = phi_update(p0, d, F[i], kap, lap, dt, nu * dt, mu);
             phi_out[i] = p1;
             tr_out[i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
         };
@@ -711,7 +918,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                 }
                 float r_;
                 if constexpr (LIT) {
-                    const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    const float M = sqrtf(mag2(g.x, g.y, g.z));
                     snx[gb][r][c] = (float)((double)g.x / ((double)M + 1e-6));
                     sny[gb][r][c] = (float)((double)g.y / ((double)M + 1e-6));
                     r_ = 0.f;
@@ -720,7 +927,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                         sgz[r - 1][c - 1] = g.z;
                     }
                 } else {
-                    const float M = __builtin_amdgcn_sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+                    const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
                     r_ = __builtin_amdgcn_rcpf(M + 1e-6f);
                     snx[gb][r][c] = g.x * r_;
                     sny[gb][r][c] = g.y * r_;
@@ -750,6 +957,8 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
     }
 }
 
+#pragma clang fp contract(fast)
+
 __global__ void transform_kernel(float* tr, const float* phi, int64_t n, int lit, float eps, float inv_eps) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         tr[i] = lit ? heav_literal(phi[i], eps) : heav_centred(phi[i], inv_eps);
@@ -761,16 +970,24 @@ Sweep1Variant s1v(const char* name) {
 }
 template <bool LIT, int TX, int TY>
 Sweep2Variant s2v(const char* name) {
-    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 1, (const void*)&sweep2_kernel<LIT, TX, TY>, nullptr, name};
+    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 1, (const void*)&sweep2_kernel<LIT, TX, TY>, nullptr, nullptr, name};
 }
 template <bool LIT, int TX, int TY, int WPE>
 Sweep2Variant s2l(const char* name) {
-    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 1, (const void*)&sweep2_lds_kernel<LIT, TX, TY, WPE>, nullptr, name};
+    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 1, (const void*)&sweep2_lds_kernel<LIT, TX, TY, WPE>, nullptr, nullptr,
+                         name};
 }
 template <bool LIT, int TX, int TY>
 Sweep2Variant s2s(const char* name) {
     return Sweep2Variant{LIT ? 1 : 0, TX, TY, 1, (const void*)&sweep2_inner_kernel<LIT, TX, TY>,
-                         (const void*)&sweep2_shell_kernel<LIT>, name};
+                         (const void*)&sweep2_shell_kernel<LIT>, nullptr, name};
+}
+// vector interior for blocks with 16-byte aligned rows, the 64x4 scalar interior for the others
+template <bool LIT, int TX, int TY, int WPE>
+Sweep2Variant s2q(const char* name) {
+    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 4, (const void*)&sweep2_vec_kernel<LIT, TX, TY, WPE>,
+                         (const void*)&sweep2_shell_kernel<LIT>, (const void*)&sweep2_inner_kernel<LIT, 64, 4>,
+                         name};
 }
 
 }  // namespace
@@ -796,11 +1013,17 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
 
 const std::vector<Sweep2Variant>& sweep2_variants() {
     static const std::vector<Sweep2Variant> v = {
+        s2q<false, 32, 8, 2>("fast_vec_32x8"),
+        s2q<false, 32, 4, 2>("fast_vec_32x4"),
+        s2q<false, 64, 4, 2>("fast_vec_64x4"),
+        s2q<false, 16, 16, 2>("fast_vec_16x16"),
+        s2q<false, 16, 8, 2>("fast_vec_16x8"),
         s2s<false, 64, 4>("fast_split_64x4"),
         s2s<false, 64, 2>("fast_split_64x2"),
         s2s<false, 32, 8>("fast_split_32x8"),
         s2v<false, 64, 4>("fast_64x4"),
         s2l<false, 32, 16, 4>("fast_lds_32x16"),
+        s2q<true, 32, 8, 2>("lit_vec_32x8"),
         s2s<true, 64, 4>("lit_split_64x4"),
         s2v<true, 64, 4>("lit_64x4"),
     };
@@ -848,6 +1071,7 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     tp.tiles1.clear();
     tp.tiles2.clear();
     tp.tiles3.clear();
+    tp.tiles4.clear();
     tp.total_voxels = 0;
     const int tx1 = V1[s1].tx, ty1 = V1[s1].ty, tx2 = V2[s2].tx, ty2 = V2[s2].ty;
     for (size_t b = 0; b < blocks.size(); ++b) {
@@ -857,14 +1081,26 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
             for (int y = 0; y < bd.h; y += ty1)
                 for (int x = 0; x < bd.w; x += tx1) tp.tiles1.push_back(Tile{(int)b, x, y, z});
         if (V2[s2].shell_fn) {
-            for (int z = 2; z < bd.l - 2; z += zc2)
-                for (int y = 2; y < bd.h - 2; y += ty2)
-                    for (int x = 2; x < bd.w - 2; x += tx2) tp.tiles2.push_back(Tile{(int)b, x, y, z});
-            const int cols = 4;
+            // vector interior when every row of the block starts on a 16-byte boundary
+            const bool vec = V2[s2].vec == 4 && (bd.off & 3) == 0 && (bd.w & 3) == 0;
+            int cols = 4;
+            if (vec) {
+                const int xend = 4 + 4 * ((bd.w - 8) >> 2);
+                cols = 4 + (bd.w - xend);
+                for (int z = 2; z < bd.l - 2; z += zc2)
+                    for (int y = 2; y < bd.h - 2; y += ty2)
+                        for (int x = 4; x < xend; x += 4 * tx2) tp.tiles2.push_back(Tile{(int)b, x, y, z});
+            } else {
+                std::vector<Tile>& dst = V2[s2].vec == 4 ? tp.tiles4 : tp.tiles2;
+                const int txs = V2[s2].vec == 4 ? 64 : tx2, tys = V2[s2].vec == 4 ? 4 : ty2;
+                for (int z = 2; z < bd.l - 2; z += zc2)
+                    for (int y = 2; y < bd.h - 2; y += tys)
+                        for (int x = 2; x < bd.w - 2; x += txs) dst.push_back(Tile{(int)b, x, y, z});
+            }
             const int64_t shell = 4 * (int64_t)bd.w * bd.h + 4 * (int64_t)bd.w * (bd.l - 4) +
                                   (int64_t)cols * (bd.h - 4) * (bd.l - 4);
             for (int64_t k = 0; k < shell; k += 256)
-                tp.tiles3.push_back(Tile{(int)b, (int)k, (int)std::min<int64_t>(256, shell - k), cols});
+                tp.tiles3.push_back(Tile{(int)b, (int)k, (int)std::min<int64_t>(256, shell - k), vec ? 1 : 0});
         } else {
             for (int z = 0; z < bd.l; z += zc2)
                 for (int y = 0; y < bd.h; y += ty2)
@@ -873,6 +1109,7 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     }
     xcd_order(tp.tiles1);
     xcd_order(tp.tiles2);
+    xcd_order(tp.tiles4);
     // Persistent grids: a few workgroups per CU, a multiple of 8 (XCD count).
     auto grid_of = [&](size_t n, int per_cu) {
         size_t g = std::min(n, (size_t)num_cus * per_cu);
@@ -882,6 +1119,7 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     tp.grid1 = grid_of(tp.tiles1.size(), 2);
     tp.grid2 = grid_of(tp.tiles2.size(), 8);
     tp.grid3 = tp.tiles3.empty() ? 0 : grid_of(tp.tiles3.size(), 8);
+    tp.grid4 = tp.tiles4.empty() ? 0 : grid_of(tp.tiles4.size(), 8);
     RSF_TRY(tp.d_tiles1.ensure(tp.tiles1.size() * sizeof(Tile)));
     RSF_TRY(tp.d_tiles2.ensure(tp.tiles2.size() * sizeof(Tile)));
     RSF_TRY(tp.d_blocks.ensure(blocks.size() * sizeof(BlockDesc)));
@@ -890,6 +1128,10 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     if (!tp.tiles3.empty()) {
         RSF_TRY(tp.d_tiles3.ensure(tp.tiles3.size() * sizeof(Tile)));
         RSF_HIP(hipMemcpy(tp.d_tiles3.p, tp.tiles3.data(), tp.tiles3.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    }
+    if (!tp.tiles4.empty()) {
+        RSF_TRY(tp.d_tiles4.ensure(tp.tiles4.size() * sizeof(Tile)));
+        RSF_HIP(hipMemcpy(tp.d_tiles4.p, tp.tiles4.data(), tp.tiles4.size() * sizeof(Tile), hipMemcpyHostToDevice));
     }
     RSF_HIP(hipMemcpy(tp.d_blocks.p, blocks.data(), blocks.size() * sizeof(BlockDesc), hipMemcpyHostToDevice));
     return RSF_OK;
@@ -919,6 +1161,14 @@ int launch_sweep2(const TilePlan& tp, const SweepArgs& a, const rsf_params* p, h
                     (void*)&p->dt,  (void*)&p->nu,   (void*)&p->mu,     (void*)&p->epsilon,
                     (void*)&inv_eps};
     if (ntiles > 0) RSF_HIP(hipLaunchKernel(v.fn, dim3(tp.grid2), dim3(v.tx * v.ty), args, 0, s));
+    if (v.scalar_fn && tp.grid4 > 0) {
+        const int n4 = (int)tp.tiles4.size();
+        void* a4[] = {(void*)&a.phi,  (void*)&a.F,     (void*)&a.phi_out, (void*)&a.tr_out,
+                      (void*)&tp.d_blocks.p, (void*)&tp.d_tiles4.p, (void*)&n4, (void*)&tp.zc2,
+                      (void*)&p->dt,  (void*)&p->nu,   (void*)&p->mu,     (void*)&p->epsilon,
+                      (void*)&inv_eps};
+        RSF_HIP(hipLaunchKernel(v.scalar_fn, dim3(tp.grid4), dim3(256), a4, 0, s));
+    }
     if (v.shell_fn && tp.grid3 > 0) {
         const int nitems = (int)tp.tiles3.size();
         void* sargs[] = {(void*)&a.phi,  (void*)&a.F,      (void*)&a.phi_out, (void*)&a.tr_out,

@@ -32,6 +32,12 @@ CASES = [
     (5.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (40, 41, 43)),
     (5.0, 1.0, rsf_amd.MODE_LITERAL, np.uint8, (40, 41, 43)),
     (1.3, 1.0, rsf_amd.MODE_FAST, np.uint8, (24, 26, 28)),   # P = 9: unfused fallback
+    # w a multiple of 4: the vector interior of sweep 2
+    (3.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (40, 44, 48)),
+    (3.0, 0.2, rsf_amd.MODE_LITERAL, np.uint8, (36, 30, 52)),
+    (1.0, 1.0, rsf_amd.MODE_FAST, np.float32, (30, 33, 36)),
+    (5.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (40, 41, 44)),
+    (3.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (23, 22, 24)),  # one row of quads per tile
 ]
 
 
@@ -171,3 +177,29 @@ def test_errors_are_reported(ctx):
     small = to_dev(phi0[:10, :10, :10].copy())
     with pytest.raises(rsf_amd.RSFError):
         ctx.step(to_dev(img[:10, :10, :10].copy()), small, torch.empty_like(small), rsf_amd.make_params(sigma=3.0))
+
+
+@pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
+def test_unaligned_buffers_match_aligned(ctx, mode):
+    """phi buffers that do not start on a 16-byte boundary take the scalar
+    interior kernel; the result matches the aligned (vector) run."""
+    import torch
+
+    shape = (36, 40, 44)
+    img, phi0, _ = scene(shape, tubes=5, seed=17)
+    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, mode=mode, iterations=1)
+    n = phi0.size
+    d_img = to_dev(img)
+    base_in = torch.zeros(n + 1, dtype=torch.float32, device="cuda")
+    base_out = torch.zeros(n + 1, dtype=torch.float32, device="cuda")
+    u_in, u_out = base_in[1:].view(shape), base_out[1:].view(shape)
+    u_in.copy_(to_dev(phi0))
+    ctx.step(d_img, u_in, u_out, p)
+    a_in, a_out = to_dev(phi0), torch.empty(shape, dtype=torch.float32, device="cuda")
+    ctx.step(d_img, a_in, a_out, p)
+    got_u, got_a = u_out.cpu().numpy(), a_out.cpu().numpy()
+    ref = oracle.step(phi0, img.astype(np.float32), oracle_params(p))
+    for got in (got_u, got_a):
+        err, tol = one_step_gate(got, ref)
+        assert err <= tol, (err, tol)
+    assert np.abs(got_u - got_a).max() <= 1e-6 * np.abs(ref).max()
