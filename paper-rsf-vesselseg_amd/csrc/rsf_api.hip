@@ -103,6 +103,7 @@ int check_params(const rsf_params* p) {
 int derive(const rsf_params* p, Derived* d) {
     RSF_TRY(check_params(p));
     RSF_TRY(taps32(p->sigma, d->taps.g, kMaxTaps, &d->P, &d->m));
+    for (int k = d->P; k < 2 * d->P - 1; ++k) d->taps.g[k] = d->taps.g[k - d->P];
     double s = 0;
     for (int k = 0; k < d->P; ++k) s += d->taps.g[k];
     d->halfS3 = (float)(0.5 * s * s * s);
@@ -244,6 +245,8 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
     };
     // no compiled variant for this P (or mode): the caller takes the unfused path
     *zc1 = *s1 >= 0 ? chunk(sweep1_variants()[*s1].tx, sweep1_variants()[*s1].ty, 2, 4 * d.P) : std::min(lmax, 128);
+    // split sweep 1: the xy kernel has no z state, short chunks balance its work items
+    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) *zc1 = std::min(lmax, 16);
     *zc2 = *s2 >= 0 ? std::min(chunk(sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, sweep2_variants()[*s2].ty,
                                      4, 16), 64)
                     : std::min(lmax, 64);
@@ -289,6 +292,17 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
     for (const auto& b : blocks)
         small = small && 4.0 * ((double)b.w * b.h * b.l + 4.0 * b.w * b.h + 4.0 * b.w + 64.0) < 4294967295.0;
     if (!small && *s2 >= 0 && sweep2_variants()[*s2].shell_fn) *s2 = find_variant2(literal ? "lit_64x4" : "fast_64x4");
+    // likewise the z kernel of the split sweep 1: take the fused kernel
+    if (!small && *s1 >= 0 && sweep1_variants()[*s1].fn2) {
+        for (size_t k = 0; k < sweep1_variants().size(); ++k) {
+            const auto& v = sweep1_variants()[k];
+            if (!v.fn2 && v.P == d.P && v.literal == literal) {
+                *s1 = (int)k;
+                *zc1 = std::min(lmax, 512);
+                break;
+            }
+        }
+    }
     // the vector interior reads 16-byte groups: caller buffers must start on a 16-byte boundary
     if (!aligned && *s2 >= 0 && sweep2_variants()[*s2].vec == 4)
         *s2 = find_variant2(literal ? "lit_split_64x4" : "fast_split_64x4");

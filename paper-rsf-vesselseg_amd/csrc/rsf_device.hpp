@@ -11,9 +11,11 @@ namespace rsf {
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kMaxTaps = 63;
 
-// Taps travel as a by-value kernel argument, so they live in SGPRs.
+// Taps travel as a by-value kernel argument, so they live in SGPRs.  g[0..P)
+// are the taps; g[P..2P-1) repeat them (g[P + i] = g[i]), so that any P
+// consecutive entries starting below P are the taps rotated.
 struct Taps {
-    float g[kMaxTaps];
+    float g[2 * kMaxTaps];
 };
 
 // One sub-volume of a (possibly batched) run: first voxel and dims.
@@ -47,6 +49,27 @@ __device__ __forceinline__ float heav_centred(float phi, float inv_eps) {
 }
 __device__ __forceinline__ float dirac_fast(float phi, float eps) {
     return (float)(1.0 / kPi) * (eps * __builtin_amdgcn_rcpf(__builtin_fmaf(phi, phi, eps * eps)));
+}
+
+// Local fits of the fast mode from bI = blur(I), bIG = blur(I*G), bG = blur(G):
+// blur(H) = (sum g)^3/2 + bG and blur(I*H) = bI/2 + bIG (H = 1/2 + G), likewise
+// for 1-H.  Written out (no implicit contraction) so that every sweep-1 kernel
+// rounds them identically.
+__device__ __forceinline__ void fast_fits(float bI, float bIG, float bG, float halfS3, float& fo, float& fi,
+                                          float& bH, float& bHc) {
+#pragma clang fp contract(off)
+    bH = halfS3 + bG;
+    bHc = halfS3 - bG;
+    fo = __builtin_fmaf(0.5f, bI, bIG) * __builtin_amdgcn_rcpf(bH);
+    fi = __builtin_fmaf(0.5f, bI, -bIG) * __builtin_amdgcn_rcpf(bHc);
+}
+// F = lambda1*Eo - lambda2*Ei, Eo = (I-fo)^2, Ei = (I-fi)^2 (sigma2 = 0,
+// kernel.cu:44-45; lambda1 == lambda2 gives e*(Eo-Ei) of kernel.cu:384).
+__device__ __forceinline__ float force_of(float I, float fo, float fi, float l1, float l2, int same) {
+#pragma clang fp contract(off)
+    const float d1 = I - fo, d2 = I - fi;
+    const float Eo = d1 * d1, Ei = d2 * d2;
+    return same ? l1 * (Eo - Ei) : (l1 * Eo - l2 * Ei);
 }
 
 // The sweep-2 kernels write their fused multiply-adds out explicitly (and
@@ -108,31 +131,6 @@ __device__ __forceinline__ void left_x(int x, int y, int z, int w, int h, int& q
     else if (y > 0) { qx = w - 1; qy = y - 1; qz = z; }
     else if (z > 0) { qx = w - 1; qy = h - 1; qz = z - 1; }
     else { qx = 0; qy = 0; qz = 0; }
-}
-
-// ------------------------------------------------ register ring (z pushes)
-// Input plane t contributes tap k to the output whose first plane is t-k;
-// slot s of the ring holds the output whose first plane is = s (mod P).
-// Each slot starts from v*g[0] and accumulates k = 1..P-1 in order, the
-// summation order of Convolution__on_Z (kernel.cu:1044-1045).
-template <int P, int NC, int S>
-__device__ __forceinline__ void zpush(float (&acc)[NC][P], const float (&v)[NC], const Taps& tp,
-                                      float (&done)[NC]) {
-#pragma unroll
-    for (int n = 0; n < NC; ++n) {
-        acc[n][S] = v[n] * tp.g[0];
-#pragma unroll
-        for (int k = 1; k < P; ++k)
-            acc[n][(S - k + P) % P] = fmaf(v[n], tp.g[k], acc[n][(S - k + P) % P]);
-        done[n] = acc[n][(S + 1) % P];
-    }
-}
-
-template <int P, int NC, int... S>
-__device__ __forceinline__ void zdispatch(int s, float (&acc)[NC][P], const float (&v)[NC],
-                                          const Taps& tp, float (&done)[NC],
-                                          std::integer_sequence<int, S...>) {
-    ((s == S ? zpush<P, NC, S>(acc, v, tp, done) : void()), ...);
 }
 
 }  // namespace rsf

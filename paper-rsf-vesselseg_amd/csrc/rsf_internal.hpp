@@ -103,8 +103,11 @@ struct Sweep1Variant {
     int P;
     int literal;
     int tx, ty, nt;
-    const void* fn;
+    const void* fn;        // fused kernel, or the xy kernel of a split variant
     const char* name;
+    const void* fn2;       // split variants: the z kernel (else null)
+    int tx2, ty2;          // z-kernel columns per work item
+    const void* fn2s[6];   // z kernel by 2 * output mode + (image is uint8)
 };
 struct Sweep2Variant {
     int literal;
@@ -122,10 +125,13 @@ struct TilePlan {
     int s1 = -1, s2 = -1;        // variant indices
     int zc1 = 0, zc2 = 0;        // z-chunk of each sweep
     int grid1 = 0, grid2 = 0;    // persistent grid sizes
-    int grid3 = 0, grid4 = 0;
-    // sweep 1; sweep 2 interior; sweep 2 shell; sweep 2 scalar interior of unaligned blocks
-    std::vector<Tile> tiles1, tiles2, tiles3, tiles4;
-    DevBuf d_tiles1, d_tiles2, d_tiles3, d_tiles4, d_blocks;
+    int grid3 = 0, grid4 = 0, grid5 = 0;
+    // sweep 1 (fused, or xy kernel); sweep 2 interior; sweep 2 shell; sweep 2 scalar
+    // interior of unaligned blocks; sweep 1 z kernel of a split variant
+    std::vector<Tile> tiles1, tiles2, tiles3, tiles4, tiles5;
+    DevBuf d_tiles1, d_tiles2, d_tiles3, d_tiles4, d_tiles5, d_blocks;
+    DevBuf xy;                   // split sweep 1: the two xy-blurred channels
+    int64_t extent = 0;          // voxels spanned by the packed buffers
     std::vector<BlockDesc> blocks;
     int64_t total_voxels = 0;
 };

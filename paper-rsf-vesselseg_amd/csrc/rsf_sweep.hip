@@ -25,16 +25,10 @@
 namespace rsf {
 namespace {
 
-// z-ring dispatch on the plane index modulo P (P <= 35): a switch, so the
-// compiler emits a short compare tree instead of a chain.
-#define RSF_ZCASE(S)                                                      \
-    case S:                                                               \
-        if constexpr (S < P) zpush<P, NC, (S < P ? S : 0)>(acc, v, taps, done); \
-        break;
-
 // WPE: minimum waves per SIMD the register allocation must allow (0 = none),
 // so that several workgroups share a CU and hide each other's barrier waits.
-template <int P, bool LIT, int TX, int TY, int NT, int WPE>
+// DB: double-buffered LDS stages, one barrier per plane.
+template <int P, bool LIT, int TX, int TY, int NT, int WPE, bool DB>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1))) void sweep1_kernel(
     const float* __restrict__ tr, const void* __restrict__ img, int img_u8,
     const float* __restrict__ blurI, float* __restrict__ out0, float* __restrict__ out1,
@@ -59,8 +53,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WP
     constexpr int NC = COLS * CH;
     constexpr int WIN = (RX + P - 1 + 3) & ~3;
 
-    __shared__ __align__(16) float sIn[CH][RH][RWP];
-    __shared__ __align__(16) float sX[CH][RH][TX];
+    constexpr int NB = DB ? 2 : 1;
+    __shared__ __align__(16) float sIn[NB][CH][RH][RWP];
+    // x-pass output rows padded by 4 floats: the y pass reads rows j and j+1 of a
+    // column in one ds_read2, which would otherwise hit the same bank
+    constexpr int TXP = TX + 4;
+    __shared__ __align__(16) float sX[NB][CH][RH][TXP];
 
     const int tid = threadIdx.x;
     const int tx = tid % TX;
@@ -117,7 +115,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WP
                 }
             }
         };
-        auto store = [&]() {
+        auto store = [&](int buf) {
 #pragma unroll
             for (int k = 0; k < PF; ++k) {
                 const int e = tid + k * NT;
@@ -150,43 +148,287 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WP
                 for (int ch = 0; ch < CH; ++ch) {
                     if constexpr (SH % 2 == 0) {
                         if (c0 >= 0 && c0 + 1 < RW)
-                            *reinterpret_cast<float2*>(&sIn[ch][r][c0]) = make_float2(cv[ch][0], cv[ch][1]);
+                            *reinterpret_cast<float2*>(&sIn[buf][ch][r][c0]) = make_float2(cv[ch][0], cv[ch][1]);
                         if (c0 + 2 >= 0 && c0 + 3 < RW)
-                            *reinterpret_cast<float2*>(&sIn[ch][r][c0 + 2]) = make_float2(cv[ch][2], cv[ch][3]);
+                            *reinterpret_cast<float2*>(&sIn[buf][ch][r][c0 + 2]) = make_float2(cv[ch][2], cv[ch][3]);
                     } else {
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            if (c0 + j >= 0 && c0 + j < RW) sIn[ch][r][c0 + j] = cv[ch][j];
+                            if (c0 + j >= 0 && c0 + j < RW) sIn[buf][ch][r][c0 + j] = cv[ch][j];
                     }
                 }
             }
         };
 
-        float acc[NC][P];
+        // one 1-D array per ring channel (a 2-D array indexed by a run-time slot
+        // would not be kept in registers)
+        static_assert(NC <= 8, "ring channels");
+        float acc0[P], acc1[P], acc2[P], acc3[P], acc4[P], acc5[P], acc6[P], acc7[P];
+        auto ring = [&](int n) -> float (&)[P] {
+            return n == 0 ? acc0 : n == 1 ? acc1 : n == 2 ? acc2 : n == 3 ? acc3
+                 : n == 4 ? acc4 : n == 5 ? acc5 : n == 6 ? acc6 : acc7;
+        };
 #pragma unroll
         for (int n = 0; n < NC; ++n)
 #pragma unroll
-            for (int k = 0; k < P; ++k) acc[n][k] = 0.f;
+            for (int k = 0; k < P; ++k) ring(n)[k] = 0.f;
 
-        fetch(0);
-        for (int t = 0; t < nplanes; ++t) {
-            store();
-            if (t + 1 < nplanes) fetch(t + 1);
-            // emission operands of this plane, loaded early
+        // emission operands (image, precomputed blur) of the output plane of plane t
+        float eI[COLS], eB[COLS];
+        auto load_emit = [&](int t) {
             const int zo = z0 + t - (P - 1);
-            float eI[COLS], eB[COLS];
-            if (t >= P - 1) {
+#pragma unroll
+            for (int cc = 0; cc < COLS; ++cc) {
+                const int x = min(x0 + tx, w - 1), y = min(y0 + ty0 + cc, h - 1);
+                const int64_t i = off + (int64_t)zo * plane + (int64_t)y * w + x;
+                eI[cc] = img_u8 ? (float)img8[i] : imgf[i];
+                eB[cc] = LIT ? 0.f : blurI[i];
+            }
+        };
+        // ---- x pass of one plane, RX consecutive outputs per work item
+        auto xpass = [&](int buf) {
+            for (int e = tid; e < CH * RH * XG; e += NT) {
+                const int ch = e / (RH * XG);
+                const int rem = e - ch * (RH * XG);
+                const int r = rem / XG;
+                const int xb = (rem - r * XG) * RX;
+                float win[WIN];
+                const float4* src = reinterpret_cast<const float4*>(&sIn[buf][ch][r][xb]);
+#pragma unroll
+                for (int q = 0; q < WIN / 4; ++q) {
+                    const float4 v4 = src[q];
+                    win[4 * q + 0] = v4.x;
+                    win[4 * q + 1] = v4.y;
+                    win[4 * q + 2] = v4.z;
+                    win[4 * q + 3] = v4.w;
+                }
+                float o4[RX];
+#pragma unroll
+                for (int o = 0; o < RX; ++o) {
+                    float a = win[o] * taps.g[0];
+#pragma unroll
+                    for (int k = 1; k < P; ++k) a = fmaf(win[o + k], taps.g[k], a);
+                    o4[o] = a;
+                }
+                *reinterpret_cast<float4*>(&sX[buf][ch][r][xb]) = make_float4(o4[0], o4[1], o4[2], o4[3]);
+            }
+        };
+        // ---- y pass of plane t for this thread's COLS rows, z ring, emission
+        auto yzemit = [&](int buf, int t) {
+            float v[NC];
+#pragma unroll
+            for (int ch = 0; ch < CH; ++ch) {
+                float col[COLS + P - 1];
+#pragma unroll
+                for (int j = 0; j < COLS + P - 1; ++j) col[j] = sX[buf][ch][ty0 + j][tx];
 #pragma unroll
                 for (int cc = 0; cc < COLS; ++cc) {
-                    const int x = min(x0 + tx, w - 1), y = min(y0 + ty0 + cc, h - 1);
-                    const int64_t i = off + (int64_t)zo * plane + (int64_t)y * w + x;
-                    eI[cc] = img_u8 ? (float)img8[i] : imgf[i];
-                    eB[cc] = LIT ? 0.f : blurI[i];
+                    float a = col[cc] * taps.g[0];
+#pragma unroll
+                    for (int k = 1; k < P; ++k) a = fmaf(col[cc + k], taps.g[k], a);
+                    v[cc * CH + ch] = a;
                 }
             }
+            // ---- z ring.  ring(n)[j] collects the output whose first plane is = j
+            // (mod P); plane t (phase s = t mod P) adds tap g[(s - j) mod P] to it,
+            // i.e. the rotated row taps.g[P-1-s .. 2P-2-s] (the taps are symmetric),
+            // and completes accumulator (s+1) mod P, which restarts from zero.  Each
+            // output thus sums g[0]*x0 + ... + g[P-1]*x(P-1) in order, the summation
+            // order of Convolution__on_Z (kernel.cu:1044-1045).  The accumulators stay
+            // in fixed registers; only the finished one is read by index (uniform).
+            const int s = t % P;
+            const float* trow = &taps.g[P - 1 - s];
+#pragma unroll
+            for (int n = 0; n < NC; ++n)
+#pragma unroll
+                for (int j = 0; j < P; ++j) ring(n)[j] = fmaf(v[n], trow[j], ring(n)[j]);
+            const int js = s == P - 1 ? 0 : s + 1;
+            float done[NC];
+#pragma unroll
+            for (int n = 0; n < NC; ++n) {
+                done[n] = ring(n)[js];
+                ring(n)[js] = 0.f;
+            }
+            if (t < P - 1) return;
+            // ---- emission of output plane zo
+            const int zo = z0 + t - (P - 1);
+            const bool zvalid = zo >= M && zo < l - M;
+#pragma unroll
+            for (int cc = 0; cc < COLS; ++cc) {
+                const int x = x0 + tx, y = y0 + ty0 + cc;
+                if (x >= w || y >= h) continue;
+                const int64_t i = off + (int64_t)zo * plane + (int64_t)y * w + x;
+                const float Iv = eI[cc];
+                float fo = 1.0f, fi = 1.0f;  // pad band: 30/30 (kernel.cu:313, main.cpp:277)
+                float bH = 30.0f, bHc = 30.0f;
+                if (zvalid && x >= M && x < w - M && y >= M && y < h - M) {
+                    if constexpr (LIT) {
+                        bH = done[cc * CH + 1];
+                        bHc = done[cc * CH + 3];
+                        fo = done[cc * CH + 0] / bH;
+                        fi = done[cc * CH + 2] / bHc;
+                    } else {
+                        fast_fits(eB[cc], done[cc * CH + 0], done[cc * CH + 1], halfS3, fo, fi, bH, bHc);
+                    }
+                }
+                if (out_mode == 0) {
+                    out0[i] = force_of(Iv, fo, fi, l1, l2, same_lambda);
+                } else {
+                    out0[i] = fo;
+                    out1[i] = fi;
+                    if (out_mode == 2) {
+                        out2[i] = bH;
+                        out3[i] = bHc;
+                    }
+                }
+            }
+        };
+
+        fetch(0);
+        if constexpr (!DB) {
+            // two barriers per plane: LDS input -> x pass -> y pass
+            for (int t = 0; t < nplanes; ++t) {
+                store(0);
+                if (t + 1 < nplanes) fetch(t + 1);
+                if (t >= P - 1) load_emit(t);
+                __syncthreads();
+                xpass(0);
+                __syncthreads();
+                yzemit(0, t);
+            }
+        } else {
+            // one barrier per plane: the x pass of plane t and the y pass, z ring and
+            // emission of plane t-1 share a barrier interval on alternating buffers
+            for (int t = 0; t <= nplanes; ++t) {
+                if (t < nplanes) {
+                    store(t & 1);
+                    if (t + 1 < nplanes) fetch(t + 1);
+                }
+                if (t >= P) load_emit(t - 1);
+                __syncthreads();
+                if (t < nplanes) xpass(t & 1);
+                if (t >= 1) yzemit((t - 1) & 1, t - 1);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 1 in two kernels (fast mode).  The fused kernel above keeps a z ring of
+// 2 x P accumulators per voxel in registers next to its LDS passes, which caps
+// its occupancy.  Here the x and y passes run alone (xy kernel, LDS only) and
+// write the two xy-blurred channels to a scratch volume; the z kernel marches
+// each (x, y) column through z with the register ring and emits F.  Same
+// operations in the same order as the fused kernel: identical results.
+
+// xy kernel: tiles of TX x TY outputs inside the valid region [M, w-M) x
+// [M, h-M) of a block, ZC planes per work item.  Per plane the (TY+2M) x
+// (TX+2M) window of I*G and G goes to LDS, the x pass writes (TY+2M) x TX
+// outputs to LDS (RX = 4 per work item), the y pass R outputs per work item
+// from R+P-1 LDS reads.  x-pass and y-pass work items are exact multiples of
+// the thread count.
+template <int P, int TX, int TY, int NT, int R, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_kernel(
+    const float* __restrict__ tr, const void* __restrict__ img, int img_u8, float* __restrict__ xyA,
+    float* __restrict__ xyB, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles,
+    int zc, Taps taps) {
+    constexpr int M = (P - 1) / 2;
+    constexpr int RH = TY + 2 * M, RW = TX + 2 * M;
+    constexpr int RWP = (RW + 3) & ~3;
+    constexpr int TXP = TX + 4;
+    constexpr int QW = RWP / 4;            // float4 groups per LDS input row
+    constexpr int NQ = RH * QW;
+    constexpr int PF = (NQ + NT - 1) / NT;
+    constexpr int RX = 4, XG = TX / RX;
+    constexpr int NXI = 2 * RH * XG;       // x-pass work items
+    constexpr int WIN = (RX + P - 1 + 3) & ~3;
+    static_assert(NXI % NT == 0, "x-pass items must be a multiple of the threads");
+    static_assert(2 * TX * (TY / R) == NT && TY % R == 0, "one y-pass item per thread");
+    __shared__ __align__(16) float sIn[2][RH][RWP];
+    __shared__ __align__(16) float sX[2][RH][TXP];
+    const int tid = threadIdx.x;
+    const uint8_t* img8 = (const uint8_t*)img;
+    const float* imgf = (const float*)img;
+    // y-pass item of this thread: channel, column, first of R rows
+    const int ych = tid / (TX * (TY / R));
+    const int yrem = tid - ych * (TX * (TY / R));
+    const int yx = yrem % TX, yr0 = (yrem / TX) * R;
+
+    for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
+        const Tile tl = tiles[item];
+        if (tl.b < 0) continue;
+        const BlockDesc bd = blocks[tl.b];
+        const int w = bd.w, h = bd.h, l = bd.l;
+        const int64_t off = bd.off, plane = (int64_t)w * h;
+        const int x0 = tl.x0, y0 = tl.y0, z0 = tl.z0;
+        const int zend = min(z0 + zc, l);
+        const bool vec = ((off | (int64_t)w) & 3) == 0 && ((uintptr_t)tr & 15) == 0 &&
+                         (img_u8 ? ((uintptr_t)img8 & 3) == 0 : ((uintptr_t)imgf & 15) == 0);
+        float4 pt[PF], pi[PF];
+        auto fetch = [&](int z) {
+            const int64_t zoff = off + (int64_t)z * plane;
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const int e = tid + k * NT;
+                if (NQ % NT != 0 && e >= NQ) break;
+                const int r = e / QW, q = e - r * QW;
+                const int gy = min(y0 - M + r, h - 1);
+                const int gx = x0 - M + 4 * q;
+                const int64_t row = zoff + (int64_t)gy * w;
+                if (vec && gx + 3 < w) {
+                    pt[k] = *reinterpret_cast<const float4*>(tr + row + gx);
+                    if (img_u8) {
+                        pi[k].x = __uint_as_float(*reinterpret_cast<const uint32_t*>(img8 + row + gx));
+                    } else {
+                        pi[k] = *reinterpret_cast<const float4*>(imgf + row + gx);
+                    }
+                } else {
+                    float tv[4], iv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int64_t idx = row + min(gx + j, w - 1);
+                        tv[j] = tr[idx];
+                        iv[j] = img_u8 ? (float)img8[idx] : imgf[idx];
+                    }
+                    pt[k] = make_float4(tv[0], tv[1], tv[2], tv[3]);
+                    pi[k] = make_float4(iv[0], iv[1], iv[2], iv[3]);
+                    if (img_u8) pi[k].x = __uint_as_float((uint32_t)iv[0] | ((uint32_t)iv[1] << 8) |
+                                                          ((uint32_t)iv[2] << 16) | ((uint32_t)iv[3] << 24));
+                }
+            }
+        };
+        auto store = [&]() {
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const int e = tid + k * NT;
+                if (NQ % NT != 0 && e >= NQ) break;
+                const int r = e / QW, q = e - r * QW;
+                float iv[4];
+                if (img_u8) {
+                    const uint32_t b4 = __float_as_uint(pi[k].x);
+                    iv[0] = (float)(b4 & 0xff);
+                    iv[1] = (float)((b4 >> 8) & 0xff);
+                    iv[2] = (float)((b4 >> 16) & 0xff);
+                    iv[3] = (float)(b4 >> 24);
+                } else {
+                    iv[0] = pi[k].x; iv[1] = pi[k].y; iv[2] = pi[k].z; iv[3] = pi[k].w;
+                }
+                // I*G and G
+                *reinterpret_cast<float4*>(&sIn[0][r][4 * q]) =
+                    make_float4(iv[0] * pt[k].x, iv[1] * pt[k].y, iv[2] * pt[k].z, iv[3] * pt[k].w);
+                *reinterpret_cast<float4*>(&sIn[1][r][4 * q]) = pt[k];
+            }
+        };
+        fetch(z0);
+        for (int z = z0; z < zend; ++z) {
+            store();
+            if (z + 1 < zend) fetch(z + 1);
             __syncthreads();
-            // ---- x pass, RX consecutive outputs per work item
-            for (int e = tid; e < CH * RH * XG; e += NT) {
+            // ---- x pass (one item at a time: registers, not latency, are the limit)
+#pragma unroll 1
+            for (int k = 0; k < NXI / NT; ++k) {
+                const int e = tid + k * NT;
                 const int ch = e / (RH * XG);
                 const int rem = e - ch * (RH * XG);
                 const int r = rem / XG;
@@ -206,80 +448,316 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WP
                 for (int o = 0; o < RX; ++o) {
                     float a = win[o] * taps.g[0];
 #pragma unroll
-                    for (int k = 1; k < P; ++k) a = fmaf(win[o + k], taps.g[k], a);
+                    for (int kk = 1; kk < P; ++kk) a = fmaf(win[o + kk], taps.g[kk], a);
                     o4[o] = a;
                 }
                 *reinterpret_cast<float4*>(&sX[ch][r][xb]) = make_float4(o4[0], o4[1], o4[2], o4[3]);
             }
             __syncthreads();
-            // ---- y pass for this thread's COLS rows
-            float v[NC];
+            // ---- y pass: R outputs of column yx, rows yr0 .. yr0+R-1
+            {
+                float col[R + P - 1];
 #pragma unroll
-            for (int ch = 0; ch < CH; ++ch) {
-                float col[COLS + P - 1];
+                for (int j = 0; j < R + P - 1; ++j) col[j] = sX[ych][yr0 + j][yx];
+                const int gx = x0 + yx;
+                float* dst = (ych == 0 ? xyA : xyB) + off + (int64_t)z * plane + gx;
 #pragma unroll
-                for (int j = 0; j < COLS + P - 1; ++j) col[j] = sX[ch][ty0 + j][tx];
+                for (int o = 0; o < R; ++o) {
+                    float a = col[o] * taps.g[0];
 #pragma unroll
-                for (int cc = 0; cc < COLS; ++cc) {
-                    float a = col[cc] * taps.g[0];
-#pragma unroll
-                    for (int k = 1; k < P; ++k) a = fmaf(col[cc + k], taps.g[k], a);
-                    v[cc * CH + ch] = a;
-                }
-            }
-            // ---- z ring
-            float done[NC];
-            switch (t % P) {
-                RSF_ZCASE(0) RSF_ZCASE(1) RSF_ZCASE(2) RSF_ZCASE(3) RSF_ZCASE(4) RSF_ZCASE(5) RSF_ZCASE(6)
-                RSF_ZCASE(7) RSF_ZCASE(8) RSF_ZCASE(9) RSF_ZCASE(10) RSF_ZCASE(11) RSF_ZCASE(12) RSF_ZCASE(13)
-                RSF_ZCASE(14) RSF_ZCASE(15) RSF_ZCASE(16) RSF_ZCASE(17) RSF_ZCASE(18) RSF_ZCASE(19) RSF_ZCASE(20)
-                RSF_ZCASE(21) RSF_ZCASE(22) RSF_ZCASE(23) RSF_ZCASE(24) RSF_ZCASE(25) RSF_ZCASE(26) RSF_ZCASE(27)
-                RSF_ZCASE(28) RSF_ZCASE(29) RSF_ZCASE(30) RSF_ZCASE(31) RSF_ZCASE(32) RSF_ZCASE(33) RSF_ZCASE(34)
-                default: break;
-            }
-            if (t < P - 1) continue;
-            // ---- emission of output plane zo
-            const bool zvalid = zo >= M && zo < l - M;
-#pragma unroll
-            for (int cc = 0; cc < COLS; ++cc) {
-                const int x = x0 + tx, y = y0 + ty0 + cc;
-                if (x >= w || y >= h) continue;
-                const int64_t i = off + (int64_t)zo * plane + (int64_t)y * w + x;
-                const float Iv = eI[cc];
-                float fo = 1.0f, fi = 1.0f;  // pad band: 30/30 (kernel.cu:313, main.cpp:277)
-                float bH = 30.0f, bHc = 30.0f;
-                if (zvalid && x >= M && x < w - M && y >= M && y < h - M) {
-                    if constexpr (LIT) {
-                        bH = done[cc * CH + 1];
-                        bHc = done[cc * CH + 3];
-                        fo = done[cc * CH + 0] / bH;
-                        fi = done[cc * CH + 2] / bHc;
-                    } else {
-                        const float bIG = done[cc * CH + 0], bG = done[cc * CH + 1];
-                        bH = halfS3 + bG;
-                        bHc = halfS3 - bG;
-                        fo = fmaf(0.5f, eB[cc], bIG) * __builtin_amdgcn_rcpf(bH);
-                        fi = fmaf(0.5f, eB[cc], -bIG) * __builtin_amdgcn_rcpf(bHc);
-                    }
-                }
-                if (out_mode == 0) {
-                    const float d1 = Iv - fo, d2 = Iv - fi;
-                    const float Eo = d1 * d1, Ei = d2 * d2;  // sigma2 = 0 (kernel.cu:44-45)
-                    out0[i] = same_lambda ? l1 * (Eo - Ei) : (l1 * Eo - l2 * Ei);
-                } else {
-                    out0[i] = fo;
-                    out1[i] = fi;
-                    if (out_mode == 2) {
-                        out2[i] = bH;
-                        out3[i] = bHc;
-                    }
+                    for (int kk = 1; kk < P; ++kk) a = fmaf(col[o + kk], taps.g[kk], a);
+                    const int gy = y0 + yr0 + o;
+                    if (gx < w - M && gy < h - M) dst[(int64_t)gy * w] = a;
                 }
             }
         }
         __syncthreads();
     }
 }
-#undef RSF_ZCASE
+
+// xy kernel, wave strips: as sweep1_xy_kernel, but each wave owns TX/NW
+// columns of the tile and runs the x pass and the y pass of its columns back
+// to back (its x-pass rows stay in a wave-private LDS strip), so only the
+// shared input window needs a workgroup barrier; that window is double
+// buffered, one barrier per plane.
+template <int P, int TX, int TY, int NW, int R, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xyw_kernel(
+    const float* __restrict__ tr, const void* __restrict__ img, int img_u8, float* __restrict__ xyA,
+    float* __restrict__ xyB, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles,
+    int zc, Taps taps) {
+    constexpr int NT = 64 * NW;
+    constexpr int M = (P - 1) / 2;
+    constexpr int RH = TY + 2 * M, RW = TX + 2 * M;
+    constexpr int RWP = (RW + 3) & ~3;
+    constexpr int QW = RWP / 4;
+    constexpr int NQ = RH * QW;
+    constexpr int PF = (NQ + NT - 1) / NT;
+    constexpr int SW = TX / NW;            // strip width (columns per wave)
+    constexpr int SWP = SW + 4;            // padded strip row
+    constexpr int RX = 4, XG = SW / RX;
+    constexpr int NXI = 2 * RH * XG;       // x-pass items per wave
+    constexpr int WIN = (RX + P - 1 + 3) & ~3;
+    static_assert(SW % RX == 0 && NXI % 64 == 0, "x-pass items must fill the wave");
+    static_assert(2 * SW * (TY / R) == 64 && TY % R == 0, "one y-pass item per lane");
+    __shared__ __align__(16) float sIn[2][2][RH][RWP];
+    __shared__ __align__(16) float sX[NW][2][RH][SWP];
+    const int tid = threadIdx.x;
+    const int wv = tid / 64, lane = tid % 64;
+    const uint8_t* img8 = (const uint8_t*)img;
+    const float* imgf = (const float*)img;
+    // y-pass item of this lane: channel, strip column, first of R rows
+    const int ych = lane / (SW * (TY / R));
+    const int yrem = lane - ych * (SW * (TY / R));
+    const int yx = yrem % SW, yr0 = (yrem / SW) * R;
+
+    for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
+        const Tile tl = tiles[item];
+        if (tl.b < 0) continue;
+        const BlockDesc bd = blocks[tl.b];
+        const int w = bd.w, h = bd.h, l = bd.l;
+        const int64_t off = bd.off, plane = (int64_t)w * h;
+        const int x0 = tl.x0, y0 = tl.y0, z0 = tl.z0;
+        const int zend = min(z0 + zc, l);
+        const bool vec = ((off | (int64_t)w) & 3) == 0 && ((uintptr_t)tr & 15) == 0 &&
+                         (img_u8 ? ((uintptr_t)img8 & 3) == 0 : ((uintptr_t)imgf & 15) == 0);
+        float4 pt[PF], pi[PF];
+        auto fetch = [&](int z) {
+            const int64_t zoff = off + (int64_t)z * plane;
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const int e = tid + k * NT;
+                if (NQ % NT != 0 && e >= NQ) break;
+                const int r = e / QW, q = e - r * QW;
+                const int gy = min(y0 - M + r, h - 1);
+                const int gx = x0 - M + 4 * q;
+                const int64_t row = zoff + (int64_t)gy * w;
+                if (vec && gx + 3 < w) {
+                    pt[k] = *reinterpret_cast<const float4*>(tr + row + gx);
+                    if (img_u8) {
+                        pi[k].x = __uint_as_float(*reinterpret_cast<const uint32_t*>(img8 + row + gx));
+                    } else {
+                        pi[k] = *reinterpret_cast<const float4*>(imgf + row + gx);
+                    }
+                } else {
+                    float tv[4], iv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int64_t idx = row + min(gx + j, w - 1);
+                        tv[j] = tr[idx];
+                        iv[j] = img_u8 ? (float)img8[idx] : imgf[idx];
+                    }
+                    pt[k] = make_float4(tv[0], tv[1], tv[2], tv[3]);
+                    pi[k] = make_float4(iv[0], iv[1], iv[2], iv[3]);
+                    if (img_u8) pi[k].x = __uint_as_float((uint32_t)iv[0] | ((uint32_t)iv[1] << 8) |
+                                                          ((uint32_t)iv[2] << 16) | ((uint32_t)iv[3] << 24));
+                }
+            }
+        };
+        auto store = [&](int b) {
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const int e = tid + k * NT;
+                if (NQ % NT != 0 && e >= NQ) break;
+                const int r = e / QW, q = e - r * QW;
+                float iv[4];
+                if (img_u8) {
+                    const uint32_t b4 = __float_as_uint(pi[k].x);
+                    iv[0] = (float)(b4 & 0xff);
+                    iv[1] = (float)((b4 >> 8) & 0xff);
+                    iv[2] = (float)((b4 >> 16) & 0xff);
+                    iv[3] = (float)(b4 >> 24);
+                } else {
+                    iv[0] = pi[k].x; iv[1] = pi[k].y; iv[2] = pi[k].z; iv[3] = pi[k].w;
+                }
+                *reinterpret_cast<float4*>(&sIn[b][0][r][4 * q]) =
+                    make_float4(iv[0] * pt[k].x, iv[1] * pt[k].y, iv[2] * pt[k].z, iv[3] * pt[k].w);
+                *reinterpret_cast<float4*>(&sIn[b][1][r][4 * q]) = pt[k];
+            }
+        };
+        fetch(z0);
+        for (int z = z0; z < zend; ++z) {
+            const int b = (z - z0) & 1;
+            store(b);
+            if (z + 1 < zend) fetch(z + 1);
+            // the input window of this plane is complete; the other buffer, read in
+            // the previous plane, is free for the next plane's store
+            __syncthreads();
+            // ---- x pass of this wave's strip (one item at a time)
+#pragma unroll 1
+            for (int k = 0; k < NXI / 64; ++k) {
+                const int e = lane + k * 64;
+                const int ch = e / (RH * XG);
+                const int rem = e - ch * (RH * XG);
+                const int r = rem / XG;
+                const int xs = (rem - r * XG) * RX;       // strip column
+                float win[WIN];
+                const float4* src = reinterpret_cast<const float4*>(&sIn[b][ch][r][wv * SW + xs]);
+#pragma unroll
+                for (int q = 0; q < WIN / 4; ++q) {
+                    const float4 v4 = src[q];
+                    win[4 * q + 0] = v4.x;
+                    win[4 * q + 1] = v4.y;
+                    win[4 * q + 2] = v4.z;
+                    win[4 * q + 3] = v4.w;
+                }
+                float o4[RX];
+#pragma unroll
+                for (int o = 0; o < RX; ++o) {
+                    float a = win[o] * taps.g[0];
+#pragma unroll
+                    for (int kk = 1; kk < P; ++kk) a = fmaf(win[o + kk], taps.g[kk], a);
+                    o4[o] = a;
+                }
+                *reinterpret_cast<float4*>(&sX[wv][ch][r][xs]) = make_float4(o4[0], o4[1], o4[2], o4[3]);
+            }
+            // the strip's x-pass rows are written by this wave alone: order the LDS
+            // writes before the reads below without a workgroup barrier
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // ---- y pass: R outputs of strip column yx, rows yr0 .. yr0+R-1
+            {
+                float col[R + P - 1];
+#pragma unroll
+                for (int j = 0; j < R + P - 1; ++j) col[j] = sX[wv][ych][yr0 + j][yx];
+                const int gx = x0 + wv * SW + yx;
+                float* dst = (ych == 0 ? xyA : xyB) + off + (int64_t)z * plane + gx;
+#pragma unroll
+                for (int o = 0; o < R; ++o) {
+                    float a = col[o] * taps.g[0];
+#pragma unroll
+                    for (int kk = 1; kk < P; ++kk) a = fmaf(col[o + kk], taps.g[kk], a);
+                    const int gy = y0 + yr0 + o;
+                    if (gx < w - M && gy < h - M) dst[(int64_t)gy * w] = a;
+                }
+            }
+            // the next x pass overwrites this wave's strip: finish the reads first
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        __syncthreads();
+    }
+}
+
+// z kernel: one thread per (x, y) column of a block, TXC x TYC columns per work
+// item, marching all l planes.  Columns inside the valid xy region run the z
+// ring over the xy-blurred channels; every voxel outside the valid region is
+// in the pad band (fits 30/30 = 1).  U8: image dtype; MODE 0 writes F, 1 the
+// fits, 2 the fits and the blurred Heavisides.  All accesses are buffer
+// operations: the column's offset in a VGPR, the plane's offset in an SGPR
+// (blocks stay below 2^32 bytes per buffer).
+template <int P, int TXC, int TYC, int D, bool U8, int MODE>
+__global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) void sweep1_z_kernel(
+    const float* __restrict__ xyA, const float* __restrict__ xyB, const void* __restrict__ img,
+    const float* __restrict__ blurI, float* __restrict__ out0, float* __restrict__ out1,
+    float* __restrict__ out2, float* __restrict__ out3, const BlockDesc* __restrict__ blocks,
+    const Tile* __restrict__ tiles, int ntiles, Taps taps, float halfS3, float l1, float l2, int same_lambda) {
+    constexpr int M = (P - 1) / 2;
+    constexpr int ESZ = U8 ? 1 : 4;
+    static_assert(P % D == 0, "P must be a multiple of the load queue depth");
+    for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
+        const Tile tl = tiles[item];
+        if (tl.b < 0) continue;
+        const BlockDesc bd = blocks[tl.b];
+        const int w = bd.w, h = bd.h, l = bd.l;
+        const int x = tl.x0 + (int)threadIdx.x % TXC, y = tl.y0 + (int)threadIdx.x / TXC;
+        if (x >= w || y >= h) continue;
+        const int plane = w * h;
+        auto rsrc = [](const void* q) {
+            return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(q), 0, (int)0xffffffff, 0x00020000);
+        };
+        const __amdgpu_buffer_rsrc_t rA = rsrc(xyA + bd.off), rB = rsrc(xyB + bd.off), rL = rsrc(blurI + bd.off);
+        const __amdgpu_buffer_rsrc_t rI = rsrc((const char*)img + bd.off * ESZ);
+        const __amdgpu_buffer_rsrc_t r0 = rsrc(out0 + bd.off), r1 = rsrc(out1 + bd.off);
+        const __amdgpu_buffer_rsrc_t r2 = rsrc(out2 + bd.off), r3 = rsrc(out3 + bd.off);
+        const int c = y * w + x;                 // element offset of the column
+        const int vc = 4 * c, vci = ESZ * c;     // byte offsets (VGPR)
+        auto ldf = [&](__amdgpu_buffer_rsrc_t r, int z) {
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vc, 4 * z * plane, 0));
+        };
+        auto ldi = [&](int z) {
+            if constexpr (U8) return (float)__builtin_amdgcn_raw_buffer_load_b8(rI, vci, z * plane, 0);
+            else return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rI, vci, 4 * z * plane, 0));
+        };
+        auto st = [&](__amdgpu_buffer_rsrc_t r, int z, float v) {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, vc, 4 * z * plane, 0);
+        };
+        auto store_out = [&](int z, float Iv, bool valid, float bI, float bIG, float bG) {
+            float fo = 1.0f, fi = 1.0f, bH = 30.0f, bHc = 30.0f;  // pad band (kernel.cu:313, main.cpp:277)
+            if (valid) fast_fits(bI, bIG, bG, halfS3, fo, fi, bH, bHc);
+            if constexpr (MODE == 0) {
+                st(r0, z, force_of(Iv, fo, fi, l1, l2, same_lambda));
+            } else {
+                st(r0, z, fo);
+                st(r1, z, fi);
+                if constexpr (MODE == 2) {
+                    st(r2, z, bH);
+                    st(r3, z, bHc);
+                }
+            }
+        };
+        auto band = [&](int z) { store_out(z, ldi(z), false, 0.f, 0.f, 0.f); };
+        const bool xyvalid = x >= M && x < w - M && y >= M && y < h - M;
+        if (!xyvalid) {
+            for (int z = 0; z < l; ++z) band(z);
+            continue;
+        }
+        for (int z = 0; z < M; ++z) band(z);
+        float acc0[P], acc1[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) acc0[j] = acc1[j] = 0.f;
+        // D planes of loads in flight: the xy channels of plane t and the image
+        // and image blur of the plane it completes, o = t - (P-1) + M.  The plane
+        // loop is unrolled by P (a multiple of D): ring and queue slots are fixed.
+        float qa[D], qb[D], qi[D], qB[D];
+        auto group = [&](int t0, auto guard_) __attribute__((always_inline)) {
+            constexpr bool GUARD = decltype(guard_)::value;
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                const int t = t0 + s;
+                if (GUARD && t >= l) break;
+                const int k = s % D;
+                const float va = qa[k], vb = qb[k], vi = qi[k], vB = qB[k];
+                const int tn = t + D;
+                if (!GUARD || tn < l) {
+                    qa[k] = ldf(rA, tn);
+                    qb[k] = ldf(rB, tn);
+                    if (!GUARD || tn >= P - 1) {
+                        qi[k] = ldi(tn - (P - 1) + M);
+                        qB[k] = ldf(rL, tn - (P - 1) + M);
+                    }
+                }
+                // plane t adds tap g[(s - j) mod P] to accumulator j and completes
+                // accumulator (s + 1) mod P (see the fused kernel's ring)
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const float g = taps.g[(s - j + P) % P];
+                    acc0[j] = fmaf(va, g, acc0[j]);
+                    acc1[j] = fmaf(vb, g, acc1[j]);
+                }
+                const int js = (s + 1) % P;
+                const float d0 = acc0[js], d1 = acc1[js];
+                acc0[js] = 0.f;
+                acc1[js] = 0.f;
+                if (!GUARD || t >= P - 1) store_out(t - (P - 1) + M, vi, true, vB, d0, d1);
+            }
+        };
+        // queue fill: planes 0 .. D-1 (their completed planes exist only from P-1 on)
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            if (k < l) {
+                qa[k] = ldf(rA, k);
+                qb[k] = ldf(rB, k);
+            }
+        }
+        group(0, std::true_type{});
+        int t0 = P;
+        for (; t0 + P + D <= l; t0 += P) group(t0, std::false_type{});
+        for (; t0 < l; t0 += P) group(t0, std::true_type{});
+        for (int z = l - M; z < l; ++z) band(z);
+    }
+}
 
 // Sweep 2 kernels: no implicit contraction.  Each kernel variant (and each
 // code path inside one, e.g. a chunk's first plane) then rounds every voxel's
@@ -964,9 +1442,35 @@ __global__ void transform_kernel(float* tr, const float* phi, int64_t n, int lit
         tr[i] = lit ? heav_literal(phi[i], eps) : heav_centred(phi[i], inv_eps);
 }
 
-template <int P, bool LIT, int TX, int TY, int NT, int WPE = 0>
+template <int P, bool LIT, int TX, int TY, int NT, int WPE = 0, bool DB = false>
 Sweep1Variant s1v(const char* name) {
-    return Sweep1Variant{P, LIT ? 1 : 0, TX, TY, NT, (const void*)&sweep1_kernel<P, LIT, TX, TY, NT, WPE>, name};
+    return Sweep1Variant{P, LIT ? 1 : 0, TX, TY, NT, (const void*)&sweep1_kernel<P, LIT, TX, TY, NT, WPE, DB>, name,
+                         nullptr, 0, 0, {}};
+}
+// split sweep 1 (fast mode): xy kernel with TX x TY tiles, z kernel with TXC x TYC columns
+template <int P, int TX, int TY, int NW, int R, int WPE, int TXC, int TYC, int D>
+Sweep1Variant s1w(const char* name) {
+    Sweep1Variant v{P, 0, TX, TY, 64 * NW, (const void*)&sweep1_xyw_kernel<P, TX, TY, NW, R, WPE>, name,
+                    (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 0>, TXC, TYC, {}};
+    v.fn2s[0] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 0>;
+    v.fn2s[1] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 0>;
+    v.fn2s[2] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 1>;
+    v.fn2s[3] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 1>;
+    v.fn2s[4] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 2>;
+    v.fn2s[5] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 2>;
+    return v;
+}
+template <int P, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D>
+Sweep1Variant s1s(const char* name) {
+    Sweep1Variant v{P, 0, TX, TY, NT, (const void*)&sweep1_xy_kernel<P, TX, TY, NT, R, WPE>, name,
+                    (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 0>, TXC, TYC, {}};
+    v.fn2s[0] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 0>;
+    v.fn2s[1] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 0>;
+    v.fn2s[2] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 1>;
+    v.fn2s[3] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 1>;
+    v.fn2s[4] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 2>;
+    v.fn2s[5] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 2>;
+    return v;
 }
 template <bool LIT, int TX, int TY>
 Sweep2Variant s2v(const char* name) {
@@ -994,14 +1498,22 @@ Sweep2Variant s2q(const char* name) {
 
 const std::vector<Sweep1Variant>& sweep1_variants() {
     static const std::vector<Sweep1Variant> v = {
+        s1s<21, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3z256"),
+        s1s<21, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
+        s1s<21, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
+        s1s<21, 128, 28, 1024, 7, 4, 256, 1, 3>("p21_fast_split_128x28w4"),
+        s1s<21, 64, 28, 512, 7, 3, 256, 1, 3>("p21_fast_split_64x28w3"),
+        s1s<21, 32, 28, 256, 7, 3, 64, 4, 3>("p21_fast_split_32x28w3q3"),
         s1v<21, false, 32, 16, 512, 4>("p21_fast_32x16w4"),
-        s1v<21, false, 32, 16, 512>("p21_fast_32x16"),
+        s1v<21, false, 32, 16, 512, 4, true>("p21_fast_32x16w4db"),
+        s1v<21, false, 16, 20, 320, 4, true>("p21_fast_16x20w4db"),
         s1v<21, false, 16, 20, 320, 4>("p21_fast_16x20w4"),
-        s1v<21, false, 16, 20, 320, 5>("p21_fast_16x20w5"),
-        s1v<21, false, 32, 20, 640, 5>("p21_fast_32x20w5"),
+        s1v<21, false, 32, 20, 640, 4, true>("p21_fast_32x20w4db"),
+        s1v<21, false, 32, 20, 640, 5, true>("p21_fast_32x20w5db"),
+        s1v<21, false, 16, 20, 320, 5, true>("p21_fast_16x20w5db"),
+        s1v<21, false, 32, 32, 1024, 4, true>("p21_fast_32x32t1kdb"),
+        s1v<21, false, 32, 16, 512>("p21_fast_32x16"),
         s1v<21, false, 32, 32, 1024, 4>("p21_fast_32x32t1k"),
-        s1v<21, false, 32, 32, 512>("p21_fast_32x32"),
-        s1v<21, false, 32, 20, 320>("p21_fast_32x20"),
         s1v<21, true, 32, 16, 512>("p21_lit_32x16"),
         s1v<7, false, 32, 32, 512>("p7_fast_32x32"),
         s1v<7, true, 32, 16, 512>("p7_lit_32x16"),
@@ -1074,12 +1586,26 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     tp.tiles4.clear();
     tp.total_voxels = 0;
     const int tx1 = V1[s1].tx, ty1 = V1[s1].ty, tx2 = V2[s2].tx, ty2 = V2[s2].ty;
+    tp.tiles5.clear();
+    tp.extent = 0;
+    const bool split1 = V1[s1].fn2 != nullptr;
+    const int M1 = (V1[s1].P - 1) / 2;
     for (size_t b = 0; b < blocks.size(); ++b) {
         const BlockDesc& bd = blocks[b];
         tp.total_voxels += (int64_t)bd.w * bd.h * bd.l;
-        for (int z = 0; z < bd.l; z += zc1)
-            for (int y = 0; y < bd.h; y += ty1)
-                for (int x = 0; x < bd.w; x += tx1) tp.tiles1.push_back(Tile{(int)b, x, y, z});
+        tp.extent = std::max(tp.extent, bd.off + (int64_t)bd.w * bd.h * bd.l);
+        if (split1) {
+            // xy tiles over the valid region, z kernel columns over the whole block
+            for (int z = 0; z < bd.l; z += zc1)
+                for (int y = M1; y < bd.h - M1; y += ty1)
+                    for (int x = M1; x < bd.w - M1; x += tx1) tp.tiles1.push_back(Tile{(int)b, x, y, z});
+            for (int y = 0; y < bd.h; y += V1[s1].ty2)
+                for (int x = 0; x < bd.w; x += V1[s1].tx2) tp.tiles5.push_back(Tile{(int)b, x, y, 0});
+        } else {
+            for (int z = 0; z < bd.l; z += zc1)
+                for (int y = 0; y < bd.h; y += ty1)
+                    for (int x = 0; x < bd.w; x += tx1) tp.tiles1.push_back(Tile{(int)b, x, y, z});
+        }
         if (V2[s2].shell_fn) {
             // vector interior when every row of the block starts on a 16-byte boundary
             const bool vec = V2[s2].vec == 4 && (bd.off & 3) == 0 && (bd.w & 3) == 0;
@@ -1120,6 +1646,11 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     tp.grid2 = grid_of(tp.tiles2.size(), 8);
     tp.grid3 = tp.tiles3.empty() ? 0 : grid_of(tp.tiles3.size(), 8);
     tp.grid4 = tp.tiles4.empty() ? 0 : grid_of(tp.tiles4.size(), 8);
+    if (split1) {
+        tp.grid1 = tp.tiles1.empty() ? 0 : grid_of(tp.tiles1.size(), 4);
+        tp.grid5 = tp.tiles5.empty() ? 0 : grid_of(tp.tiles5.size(), 8);
+        RSF_TRY(tp.xy.ensure((size_t)tp.extent * 2 * sizeof(float)));
+    }
     RSF_TRY(tp.d_tiles1.ensure(tp.tiles1.size() * sizeof(Tile)));
     RSF_TRY(tp.d_tiles2.ensure(tp.tiles2.size() * sizeof(Tile)));
     RSF_TRY(tp.d_blocks.ensure(blocks.size() * sizeof(BlockDesc)));
@@ -1128,6 +1659,10 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     if (!tp.tiles3.empty()) {
         RSF_TRY(tp.d_tiles3.ensure(tp.tiles3.size() * sizeof(Tile)));
         RSF_HIP(hipMemcpy(tp.d_tiles3.p, tp.tiles3.data(), tp.tiles3.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    }
+    if (!tp.tiles5.empty()) {
+        RSF_TRY(tp.d_tiles5.ensure(tp.tiles5.size() * sizeof(Tile)));
+        RSF_HIP(hipMemcpy(tp.d_tiles5.p, tp.tiles5.data(), tp.tiles5.size() * sizeof(Tile), hipMemcpyHostToDevice));
     }
     if (!tp.tiles4.empty()) {
         RSF_TRY(tp.d_tiles4.ensure(tp.tiles4.size() * sizeof(Tile)));
@@ -1143,6 +1678,25 @@ int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, cons
     if (v.P != d.P) return set_error(RSF_ERR_INVALID_ARGUMENT, "sweep-1 variant built for P=%d, run has P=%d", v.P, d.P);
     const int ntiles = (int)tp.tiles1.size();
     const int same = (p->lambda1 == p->lambda2) ? 1 : 0;
+    if (v.fn2) {
+        if (a.img_u8 != 0 && a.img_u8 != 1) return set_error(RSF_ERR_INVALID_ARGUMENT, "image dtype");
+        float* xyA = tp.xy.as<float>();
+        float* xyB = xyA + tp.extent;
+        if (ntiles > 0) {
+            void* a1[] = {(void*)&a.tr, (void*)&a.img, (void*)&a.img_u8, (void*)&xyA, (void*)&xyB,
+                          (void*)&tp.d_blocks.p, (void*)&tp.d_tiles1.p, (void*)&ntiles, (void*)&tp.zc1,
+                          (void*)&d.taps};
+            RSF_HIP(hipLaunchKernel(v.fn, dim3(tp.grid1), dim3(v.nt), a1, 0, s));
+        }
+        const int n5 = (int)tp.tiles5.size();
+        if (a.out_fits < 0 || a.out_fits > 2) return set_error(RSF_ERR_INVALID_ARGUMENT, "sweep-1 output mode");
+        void* a2[] = {(void*)&xyA,      (void*)&xyB,      (void*)&a.img,    (void*)&a.blurI,
+                      (void*)&a.out0,   (void*)&a.out1,   (void*)&a.out2,   (void*)&a.out3,
+                      (void*)&tp.d_blocks.p, (void*)&tp.d_tiles5.p, (void*)&n5, (void*)&d.taps,
+                      (void*)&d.halfS3, (void*)&p->lambda1, (void*)&p->lambda2, (void*)&same};
+        RSF_HIP(hipLaunchKernel(v.fn2s[2 * a.out_fits + a.img_u8], dim3(tp.grid5), dim3(v.tx2 * v.ty2), a2, 0, s));
+        return RSF_OK;
+    }
     void* args[] = {(void*)&a.tr,       (void*)&a.img,     (void*)&a.img_u8, (void*)&a.blurI,
                     (void*)&a.out0,     (void*)&a.out1,    (void*)&a.out2,   (void*)&a.out3,
                     (void*)&a.out_fits, (void*)&tp.d_blocks.p,
