@@ -246,7 +246,7 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
     // no compiled variant for this P (or mode): the caller takes the unfused path
     *zc1 = *s1 >= 0 ? chunk(sweep1_variants()[*s1].tx, sweep1_variants()[*s1].ty, 2, 4 * d.P) : std::min(lmax, 128);
     // split sweep 1: the xy kernel has no z state, short chunks balance its work items
-    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) *zc1 = std::min(lmax, 16);
+    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) *zc1 = std::min(lmax, 32);
     *zc2 = *s2 >= 0 ? std::min(chunk(sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, sweep2_variants()[*s2].ty,
                                      4, 16), 64)
                     : std::min(lmax, 64);

@@ -1498,9 +1498,9 @@ Sweep2Variant s2q(const char* name) {
 
 const std::vector<Sweep1Variant>& sweep1_variants() {
     static const std::vector<Sweep1Variant> v = {
+        s1s<21, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3z256"),
         s1s<21, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
-        s1s<21, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, 128, 28, 1024, 7, 4, 256, 1, 3>("p21_fast_split_128x28w4"),
         s1s<21, 64, 28, 512, 7, 3, 256, 1, 3>("p21_fast_split_64x28w3"),
         s1s<21, 32, 28, 256, 7, 3, 64, 4, 3>("p21_fast_split_32x28w3q3"),
@@ -1525,9 +1525,9 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
 
 const std::vector<Sweep2Variant>& sweep2_variants() {
     static const std::vector<Sweep2Variant> v = {
+        s2q<false, 64, 4, 2>("fast_vec_64x4"),
         s2q<false, 32, 8, 2>("fast_vec_32x8"),
         s2q<false, 32, 4, 2>("fast_vec_32x4"),
-        s2q<false, 64, 4, 2>("fast_vec_64x4"),
         s2q<false, 16, 16, 2>("fast_vec_16x16"),
         s2q<false, 16, 8, 2>("fast_vec_16x8"),
         s2s<false, 64, 4>("fast_split_64x4"),
