@@ -127,6 +127,90 @@ def batched_run(rsf_amd, torch, ctx, steps, mode, nblocks=64):
     }
 
 
+def large_run(rsf_amd, torch, ctx, steps, mode):
+    """1024x1024x768 uint8 volume (2000 tubes, radius 1.5-8, noise sigma 20, seed 5), the
+    default kernels and the autotuned kernels side by side."""
+    l, h, w = 768, 1024, 1024
+    t0 = time.perf_counter()
+    img, _ = rsf_amd.synth_tubes((l, h, w), tubes=2000, rmin=1.5, rmax=8.0, bias=20.0, noise=20.0, seed=5)
+    phi0 = rsf_amd.initial_phi(img, 1.0, 110.0)
+    log(f"generated {w}x{h}x{l} phantom in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    d_img = torch.from_numpy(img).to(dev)
+    d_phi0 = torch.from_numpy(phi0).to(dev)
+    del img, phi0
+    d_phi = d_phi0.clone()
+    blocks = [(0, w, h, l)]
+    vox = float(w) * h * l
+    out = {"volume": [w, h, l], "iterations": steps}
+    for label in ("plain", "autotuned"):
+        choice = None
+        if label == "autotuned":
+            choice = ctx.autotune(blocks, rsf_amd.DTYPE_U8, params(rsf_amd, 1, mode))
+            log(f"large autotune: {choice}")
+        d_phi.copy_(d_phi0)
+        ctx.run(d_img, d_phi, params(rsf_amd, 2, mode))   # workspace, tile plans, image blur
+        d_phi.copy_(d_phi0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loop_ms = ctx.run(d_img, d_phi, params(rsf_amd, steps, mode))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[label] = {"value": round(vox * steps / wall / 1e9, 3), "ms_per_iteration": round(wall * 1e3 / steps, 4),
+                      "loop_ms_events": round(loop_ms, 1), "kernels": choice or "defaults"}
+        log(f"large {label}: {wall:.2f} s, {out[label]['value']} Gvox-it/s")
+    out["inside_fraction"] = round(float((d_phi < 0).float().mean().item()), 5)
+    return out
+
+
+def adjoint_run(rsf_amd, torch, ctx, K=32, C=8, nvol=8, n=128):
+    """Differentiated batch: nvol volumes of n^3 (seeds 4000+b), K unrolled steps with
+    checkpoints every C, BCE of sigmoid(-phi/eps) against the generator's masks; forward
+    and backward timed separately."""
+    import numpy as np
+
+    from rsf_amd.autograd import bce_inside_loss, rsf_unrolled
+
+    shapes = [(n, n, n)] * nvol
+    blocks, total = rsf_amd.pack_blocks(shapes)
+    img = np.empty(total, np.uint8)
+    phi = np.empty(total, np.float32)
+    mask = np.empty(total, np.float32)
+    for b, (off, w, h, l) in enumerate(blocks):
+        bi, bm = rsf_amd.synth_tubes((n, n, n), tubes=40, rmin=2.0, rmax=6.0, bias=20.0, noise=15.0, seed=4000 + b)
+        img[off:off + n ** 3] = bi.ravel()
+        mask[off:off + n ** 3] = bm.ravel()
+        phi[off:off + n ** 3] = rsf_amd.initial_phi(bi, 1.0, 110.0).ravel()
+    dev = torch.device("cuda:0")
+    d_img = torch.from_numpy(img).to(dev)
+    d_mask = torch.from_numpy(mask).to(dev)
+    theta = torch.tensor([1.0, 1.0, 65.025, 1.0], device=dev, requires_grad=True)
+    d_phi = torch.from_numpy(phi).to(dev).requires_grad_(True)
+    times = {}
+    for rep in range(2):   # the first repetition builds plans and workspaces
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = rsf_unrolled(d_phi, theta, d_img, blocks, K, C, sigma=3.0, epsilon=1.0, context=ctx)
+        loss = bce_inside_loss(out, d_mask, 1.0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        times = {"forward_s": t1 - t0, "backward_s": t2 - t1}
+        d_phi.grad = None
+        theta.grad = None
+    vs = total * K
+    log(f"adjoint: forward {times['forward_s'] * 1e3:.1f} ms, backward {times['backward_s'] * 1e3:.1f} ms")
+    return {
+        "volumes": nvol, "shape": [n, n, n], "unrolled_steps": K, "checkpoint_every": C,
+        "forward_ms": round(times["forward_s"] * 1e3, 2), "backward_ms": round(times["backward_s"] * 1e3, 2),
+        "forward_Gvox_steps_per_s": round(vs / times["forward_s"] / 1e9, 3),
+        "backward_Gvox_steps_per_s": round(vs / times["backward_s"] / 1e9, 3),
+        "loss": round(float(loss.detach()), 6),
+    }
+
+
 def cpu_baseline(img, phi0, budget_s):
     import numpy as np
 
@@ -178,6 +262,11 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--large", action="store_true",
+                    help="also run the 1024x1024x768 volume (2000 tubes), default and autotuned kernels")
+    ap.add_argument("--large-steps", type=int, default=1000)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time the differentiated batch (8 x 128^3, 32 steps, checkpoints every 8)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scale for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -254,6 +343,10 @@ def main():
     }
     if not args.no_batched:
         line["batched"] = batched_run(rsf_amd, torch, ctx, args.batched_steps, mode)
+    if args.adjoint:
+        line["adjoint"] = adjoint_run(rsf_amd, torch, ctx)
+    if args.large:
+        line["large"] = large_run(rsf_amd, torch, ctx, args.large_steps, mode)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(r["img"], r["phi0"], args.cpu_budget)
     ctx.close()

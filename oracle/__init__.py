@@ -63,6 +63,7 @@ def lib():
         cf = C.c_float
         sig = {
             "oracle_taps": (ci, [cf, f32p, ci, C.POINTER(ci), C.POINTER(ci)]),
+            "oracle_atan_f64": (None, [f64p, f64p, i64]),
             "oracle_heaviside": (None, [f32p, f32p, i64, cf]),
             "oracle_deri_heaviside": (None, [f32p, f32p, i64, cf]),
             "oracle_multiplication": (None, [f32p, f32p, f32p, i64]),
@@ -111,6 +112,13 @@ def taps(sigma: float):
     if rc:
         raise ValueError(f"sigma={sigma}: unsupported (floor(7*sigma) even or too small)")
     return g[: P.value].copy(), P.value, m.value
+
+
+def atan_f64(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    lib().oracle_atan_f64(out, x, x.size)
+    return out
 
 
 def heaviside(phi, eps=0.2):

@@ -31,10 +31,14 @@
  *   step_f64             same                      agrees with the FP32 step to FP32 rounding
  *   step_adjoint_f64     derivative of step_f64    FP64 central finite differences; transpose
  *                                                  identities <Df,u> = <f,D^T u>
+ *   det_atan             (atanf of kernel.cu:258)  numpy arctan: relative error < 4e-16, and
+ *                                                  the float rounding equals numpy's on dense
+ *                                                  samples
  * Parity unpinned (no reference output exists for them): the value the
  * reference reads for voxel (0,0,0) at index -1 (defined here as the clamp,
  * i.e. voxel 0 itself), nvcc's choice of FMA contraction outside the
- * correlation loops, and CUDA's atanf/sqrtf/expf last-ulp results.
+ * correlation loops, and CUDA's atanf/sqrtf/expf last-ulp results (the
+ * oracle uses the correctly rounded atanf, IEEE sqrtf and glibc expf).
  */
 #include "rsf_oracle.h"
 
@@ -77,10 +81,44 @@ int taps_impl(float sigma, float* g, int cap, int* P_out, int* m_out) {
 
 /* ---------------------------------------------------------------- FP32 ops */
 
+/* The float arctangent.  CUDA's atanf (kernel.cu:258) is a vendor routine whose
+ * last-ulp results no output of the reference pins; the oracle takes the
+ * correctly rounded atanf instead: an arctangent in double precision from IEEE
+ * +, -, *, / only (the rational approximation of S. L. Moshier's Cephes atan.c
+ * after its range reduction: x > tan(3pi/8): pi/2 - atan(1/x); x > 0.66:
+ * pi/4 + atan((x-1)/(x+1))), rounded once to float.  Built without
+ * contraction, so any IEEE machine evaluating this sequence gets these bits.
+ * Pinned against numpy's arctan (tests/test_oracle.py). */
+double det_atan(double x) {
+    const double ax = x < 0 ? -x : x;
+    double y, v;
+    int flag;
+    if (ax > 2.41421356237309504880) {          /* tan(3 pi / 8) */
+        y = 1.57079632679489661923; flag = 1; v = -(1.0 / ax);
+    } else if (ax <= 0.66) {
+        y = 0.0; flag = 0; v = ax;
+    } else {
+        y = 7.85398163397448309616e-1; flag = 2; v = (ax - 1.0) / (ax + 1.0);
+    }
+    const double z = v * v;
+    const double pz = (((-8.750608600031904122785e-1 * z - 1.615753718733365076637e1) * z
+                        - 7.500855792314704667340e1) * z - 1.228866684490136173410e2) * z
+                      - 6.485021904942025371773e1;
+    const double qz = ((((z + 2.485846490142306297962e1) * z + 1.650270098316988542046e2) * z
+                        + 4.328810604912902668951e2) * z + 4.853903996359136964868e2) * z
+                      + 1.945506571482613964425e2;
+    double r = v * (z * pz / qz) + v;
+    if (flag == 2) r = r + 0.5 * 6.123233995736765886130e-17;
+    else if (flag == 1) r = r + 6.123233995736765886130e-17;
+    y = y + r;
+    return x < 0 ? -y : y;
+}
+inline float det_atanf(float x) { return (float)det_atan((double)x); }
+
 void heaviside_f(float* out, const float* in, int64_t n, float eps) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++)   // kernel.cu:258-259: double 2/pi, float atan
-        out[i] = (float)(0.5 * (1 + (2 / kPi) * (double)atanf(in[i] / eps)));
+        out[i] = (float)(0.5 * (1 + (2 / kPi) * (double)det_atanf(in[i] / eps)));
 }
 
 void deri_heaviside_f(float* out, const float* in, int64_t n, float eps) {
@@ -510,6 +548,9 @@ void oracle_division(float* out, const float* a, const float* b, int64_t n) {
 void oracle_division_grad(float* out, const float* g, const float* M, int64_t n) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) out[i] = (float)(g[i] / (M[i] + 1e-6));   // kernel.cu:185
+}
+void oracle_atan_f64(double* out, const double* x, int64_t n) {
+    for (int64_t i = 0; i < n; i++) out[i] = det_atan(x[i]);
 }
 void oracle_gradientmagnitude(float* M, const float* gx, const float* gy, const float* gz, int64_t n) {
 #pragma omp parallel for schedule(static)

@@ -43,6 +43,10 @@ typedef struct oracle_params {
  * kernel.cu:1067) or when cap is too small. */
 int oracle_taps(float sigma, float* g, int cap, int* P, int* m);
 
+/* The arctangent behind the oracle's float atanf: double precision from IEEE
+ * basic operations only (Cephes rational approximation). */
+void oracle_atan_f64(double* out, const double* x, int64_t n);
+
 /* Element-wise kernels, n voxels. */
 void oracle_heaviside(float* out, const float* phi, int64_t n, float eps);
 void oracle_deri_heaviside(float* out, const float* phi, int64_t n, float eps);

@@ -31,12 +31,47 @@ struct Tile {
 };
 
 // ---------------------------------------------------------------- point ops
+// Arctangent in double precision from IEEE additions, multiplications and
+// divisions only: the rational approximation of S. L. Moshier's Cephes
+// library (atan.c) after its range reduction to |x| <= 0.66 (x > tan(3pi/8):
+// pi/2 - atan(1/x); x > 0.66: pi/4 + atan((x-1)/(x+1))).  Without contraction
+// every step rounds as on any IEEE machine, so the host oracle, which evaluates
+// the same sequence, gets the same bits.  Relative error below 2e-16.
+__device__ __forceinline__ double det_atan(double x) {
+#pragma clang fp contract(off)
+    const double ax = x < 0 ? -x : x;
+    double y, v;
+    int flag;
+    if (ax > 2.41421356237309504880) {          // tan(3 pi / 8)
+        y = 1.57079632679489661923; flag = 1; v = -(1.0 / ax);
+    } else if (ax <= 0.66) {
+        y = 0.0; flag = 0; v = ax;
+    } else {
+        y = 7.85398163397448309616e-1; flag = 2; v = (ax - 1.0) / (ax + 1.0);
+    }
+    const double z = v * v;
+    const double pz = (((-8.750608600031904122785e-1 * z - 1.615753718733365076637e1) * z
+                        - 7.500855792314704667340e1) * z - 1.228866684490136173410e2) * z
+                      - 6.485021904942025371773e1;
+    const double qz = ((((z + 2.485846490142306297962e1) * z + 1.650270098316988542046e2) * z
+                        + 4.328810604912902668951e2) * z + 4.853903996359136964868e2) * z
+                      + 1.945506571482613964425e2;
+    double r = v * (z * pz / qz) + v;
+    if (flag == 2) r = r + 0.5 * 6.123233995736765886130e-17;
+    else if (flag == 1) r = r + 6.123233995736765886130e-17;
+    y = y + r;
+    return x < 0 ? -y : y;
+}
+// The float arctangent of the literal path: det_atan rounded once to float,
+// i.e. the correctly rounded atanf except in (exceedingly rare) ties.
+__device__ __forceinline__ float det_atanf(float x) { return (float)det_atan((double)x); }
+
 // Heaviside of CUDA_RSF/kernel.cu:258-259: float atan, double 2/pi mul-add.
 // Contraction is off inside the literal helpers so that each operation is
 // rounded on its own, as in the oracle.
 __device__ __forceinline__ float heav_literal(float phi, float eps) {
 #pragma clang fp contract(off)
-    return (float)(0.5 * (1.0 + (2.0 / kPi) * (double)atanf(phi / eps)));
+    return (float)(0.5 * (1.0 + (2.0 / kPi) * (double)det_atanf(phi / eps)));
 }
 // Dirac of kernel.cu:275-276: float quotient scaled by the double 1/pi.
 __device__ __forceinline__ float dirac_literal(float phi, float eps) {
@@ -80,18 +115,29 @@ __device__ __forceinline__ float force_of(float I, float fo, float fi, float l1,
 __device__ __forceinline__ float half_sum3(float a, float b, float c) {
     return __builtin_fmaf(c, 0.5f, __builtin_fmaf(a, 0.5f, b * 0.5f));
 }
-// |g|^2 = gx^2 + gy^2 + gz^2 (kernel.cu:914-933).
+// |g|^2 = gx^2 + gy^2 + gz^2 (kernel.cu:914-933).  The literal path rounds each
+// product and sum on its own, left to right, as the reference kernel reads; the
+// fast path fuses two multiply-adds.
+template <bool LIT>
 __device__ __forceinline__ float mag2(float gx, float gy, float gz) {
-    return __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
+#pragma clang fp contract(off)
+    if constexpr (LIT) return (gx * gx + gy * gy) + gz * gz;
+    else return __builtin_fmaf(gz, gz, __builtin_fmaf(gy, gy, gx * gx));
 }
 // phi' = ((phi - (delta*F)*dt) + ((nu*dt)*delta)*kappa) + mu*(dt*(L - kappa))
-// (kernel.cu:384-386, F = lambda1*Eo - lambda2*Ei); nudt = nu*dt.
+// (kernel.cu:384-386, F = lambda1*Eo - lambda2*Ei); nudt = nu*dt.  Literal:
+// every operation rounded on its own; fast: three fused multiply-adds.
+template <bool LIT>
 __device__ __forceinline__ float phi_update(float p0, float d, float F, float kap, float lap, float dt, float nudt,
                                             float mu) {
 #pragma clang fp contract(off)
-    const float a = __builtin_fmaf(-(d * F), dt, p0);
-    const float b = __builtin_fmaf(nudt * d, kap, a);
-    return __builtin_fmaf(mu, dt * (lap - kap), b);
+    if constexpr (LIT) {
+        return ((p0 - (d * F) * dt) + (nudt * d) * kap) + mu * (dt * (lap - kap));
+    } else {
+        const float a = __builtin_fmaf(-(d * F), dt, p0);
+        const float b = __builtin_fmaf(nudt * d, kap, a);
+        return __builtin_fmaf(mu, dt * (lap - kap), b);
+    }
 }
 
 // ------------------------------------------------- difference operator rules

@@ -788,7 +788,7 @@ __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const 
     float nRx, nLx, nRy, nLy, nRz, nLz;
     if constexpr (LIT) {
         auto nrm = [](const Grad& g, float c) {
-            const float M = sqrtf(mag2(g.x, g.y, g.z));
+            const float M = sqrtf(mag2<LIT>(g.x, g.y, g.z));
             return (float)((double)c / ((double)M + 1e-6));
         };
         nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -796,7 +796,7 @@ __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const 
         nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
     } else {
         auto nrm = [](const Grad& g, float c) {
-            const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
+            const float M = __builtin_amdgcn_sqrtf(mag2<false>(g.x, g.y, g.z));
             return c * __builtin_amdgcn_rcpf(M + 1e-6f);
         };
         nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -807,7 +807,7 @@ __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const 
     const float lap = half_sum3(gRx.x - gLx.x, gRy.y - gLy.y, gRz.z - gLz.z);
     const float p0 = f[i];
     const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
-    const float p1 = phi_update(p0, d, F[i], kap, lap, dt, nu * dt, mu);
+    const float p1 = phi_update<LIT>(p0, d, F[i], kap, lap, dt, nu * dt, mu);
     phi_out[i] = p1;
     tr_out[i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
 }
@@ -922,7 +922,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
             float nRx, nLx, nRy, nLy, nRz, nLz;
             if constexpr (LIT) {
                 auto nrm = [](const Grad& g, float cc) {
-                    const float M = sqrtf(mag2(g.x, g.y, g.z));
+                    const float M = sqrtf(mag2<LIT>(g.x, g.y, g.z));
                     return (float)((double)cc / ((double)M + 1e-6));
                 };
                 nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -930,7 +930,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
                 nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
             } else {
                 auto nrm = [](const Grad& g, float cc) {
-                    const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
+                    const float M = __builtin_amdgcn_sqrtf(mag2<false>(g.x, g.y, g.z));
                     return cc * __builtin_amdgcn_rcpf(M + 1e-6f);
                 };
                 nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -940,7 +940,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
             const float kap = half_sum3(nRx - nLx, nRy - nLy, nRz - nLz);
             const float lap = half_sum3(gRx.x - gLx.x, gRy.y - gLy.y, gRz.z - gLz.z);
             const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
-            const float p1 = phi_update(c0, d, r[19], kap, lap, dt, nu * dt, mu);
+            const float p1 = phi_update<LIT>(c0, d, r[19], kap, lap, dt, nu * dt, mu);
             const float t1 = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, p1), rO, (int)v, 0, 0);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, t1), rT, (int)v, 0, 0);
@@ -983,10 +983,10 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 template <bool LIT>
 __device__ __forceinline__ float norm_comp(float c, float gx, float gy, float gz) {
     if constexpr (LIT) {
-        const float M = sqrtf(mag2(gx, gy, gz));
+        const float M = sqrtf(mag2<true>(gx, gy, gz));
         return (float)((double)c / ((double)M + 1e-6));
     } else {
-        const float M = __builtin_amdgcn_sqrtf(mag2(gx, gy, gz));
+        const float M = __builtin_amdgcn_sqrtf(mag2<false>(gx, gy, gz));
         return c * __builtin_amdgcn_rcpf(M + 1e-6f);
     }
 }
@@ -1136,7 +1136,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                 const float lap = half_sum3(gxs[i + 2] - gxs[i], gyu - gyd, GZ[kp1][i] - GZ[km1][i]);
                 const float c0 = C0[i + 2];
                 const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
-                p1[i] = phi_update(c0, d, FF[q][i], kap, lap, dt, nu * dt, mu);
+                p1[i] = phi_update<LIT>(c0, d, FF[q][i], kap, lap, dt, nu * dt, mu);
                 t1[i] = LIT ? heav_literal(p1[i], eps) : heav_centred(p1[i], inv_eps);
             }
             u32x4 po, pt;
@@ -1225,7 +1225,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
                 float nRx, nLx, nRy, nLy, nRz, nLz;
                 if constexpr (LIT) {
                     auto nrm = [](const Grad& g, float cc) {
-                        const float M = sqrtf(mag2(g.x, g.y, g.z));
+                        const float M = sqrtf(mag2<LIT>(g.x, g.y, g.z));
                         return (float)((double)cc / ((double)M + 1e-6));
                     };
                     nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -1233,7 +1233,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
                     nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
                 } else {
                     auto nrm = [](const Grad& g, float cc) {
-                        const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
+                        const float M = __builtin_amdgcn_sqrtf(mag2<false>(g.x, g.y, g.z));
                         return cc * __builtin_amdgcn_rcpf(M + 1e-6f);
                     };
                     nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -1244,7 +1244,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
                 const float lap = half_sum3(gRx.x - gLx.x, gRy.y - gLy.y, gRz.z - gLz.z);
                 const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
                 const float Fv = F[bd.off + i];
-                const float p1 = phi_update(c0, d, Fv, kap, lap, dt, nu * dt, mu);
+                const float p1 = phi_update<LIT>(c0, d, Fv, kap, lap, dt, nu * dt, mu);
                 phi_out[bd.off + i] = p1;
                 tr_out[bd.off + i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
                 continue;
@@ -1262,7 +1262,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
             float nRx, nLx, nRy, nLy, nRz, nLz;
             if constexpr (LIT) {
                 auto nrm = [](const Grad& g, float c) {
-                    const float M = sqrtf(mag2(g.x, g.y, g.z));
+                    const float M = sqrtf(mag2<LIT>(g.x, g.y, g.z));
                     return (float)((double)c / ((double)M + 1e-6));
                 };
                 nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -1270,7 +1270,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
                 nRz = nrm(gRz, gRz.z); nLz = nrm(gLz, gLz.z);
             } else {
                 auto nrm = [](const Grad& g, float c) {
-                    const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
+                    const float M = __builtin_amdgcn_sqrtf(mag2<false>(g.x, g.y, g.z));
                     return c * __builtin_amdgcn_rcpf(M + 1e-6f);
                 };
                 nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
@@ -1282,7 +1282,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
             const float p0 = f[i];
             const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
             const float Fv = F[bd.off + i];
-            const float p1 = phi_update(p0, d, Fv, kap, lap, dt, nu * dt, mu);
+            const float p1 = phi_update<LIT>(p0, d, Fv, kap, lap, dt, nu * dt, mu);
             phi_out[bd.off + i] = p1;
             tr_out[bd.off + i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
         }
@@ -1356,7 +1356,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
             const float p0 = sp[z % 3][ty + 2][tx + 2];
             const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
             const int64_t i = bd.off + (int64_t)z * plane + (int64_t)y * w + x;
-            const float p1 = phi_update(p0, d, F[i], kap, lap, dt, nu * dt, mu);
+            const float p1 = phi_update<LIT>(p0, d, F[i], kap, lap, dt, nu * dt, mu);
             phi_out[i] = p1;
             tr_out[i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
         };
@@ -1396,7 +1396,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                 }
                 float r_;
                 if constexpr (LIT) {
-                    const float M = sqrtf(mag2(g.x, g.y, g.z));
+                    const float M = sqrtf(mag2<LIT>(g.x, g.y, g.z));
                     snx[gb][r][c] = (float)((double)g.x / ((double)M + 1e-6));
                     sny[gb][r][c] = (float)((double)g.y / ((double)M + 1e-6));
                     r_ = 0.f;
@@ -1405,7 +1405,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                         sgz[r - 1][c - 1] = g.z;
                     }
                 } else {
-                    const float M = __builtin_amdgcn_sqrtf(mag2(g.x, g.y, g.z));
+                    const float M = __builtin_amdgcn_sqrtf(mag2<false>(g.x, g.y, g.z));
                     r_ = __builtin_amdgcn_rcpf(M + 1e-6f);
                     snx[gb][r][c] = g.x * r_;
                     sny[gb][r][c] = g.y * r_;

@@ -1,7 +1,7 @@
-"""Device engine against the stored golden vectors (tests/golden/rsf_small.npz),
-both modes: after one step max|dphi| <= 1e-4 * max|phi| (the gate) and, for
-the literal mode, the FP32 rounding bound 2e-6; after T > 1 steps the sign
-masks agree and the values stay within 1e-3 * max|phi|."""
+"""Device engine against the stored golden vectors (tests/golden/rsf_small.npz):
+the literal mode reproduces them bit for bit; the fast mode meets the one-step
+gate max|dphi| <= 1e-4 * max|phi| and, after T > 1 steps, identical sign masks
+with values within 1e-3 * max|phi|."""
 import numpy as np
 import pytest
 
@@ -42,11 +42,11 @@ def test_engine_matches_golden(ctx, name, T, mode):
     got = out.cpu().numpy()
     scale = float(np.abs(ref).max())
     err = float(np.abs(got.astype(np.float64) - ref).max())
-    if T == 1:
+    if mode == rsf_amd.MODE_LITERAL:
+        assert np.array_equal(got, ref), err / scale      # bit-identical arithmetic
+    elif T == 1:
         e, tol = one_step_gate(got, ref)
         assert e <= tol, (e, tol)
-        if mode == rsf_amd.MODE_LITERAL:
-            assert err <= 2e-6 * scale, err / scale
     else:
         assert err <= 1e-3 * scale, err / scale
         assert int(np.count_nonzero((got > 0) != (ref > 0))) == 0

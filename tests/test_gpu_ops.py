@@ -37,9 +37,8 @@ def test_mirror_heaviside_and_dirac():
     phi.ravel()[:3] = [0.0, 10.0, -10.0]
     out = np.empty_like(phi)
     rsf_amd.check(rsf_amd.lib().rsf_heaviside_cal(_p(out), _p(phi), *_dims()))
-    # device atanf vs glibc atanf may differ by an ulp of atan (<= 1.2e-7), which
-    # 0.5 + atan/pi carries into H as an absolute error
-    assert float(np.abs(out - oracle.heaviside(phi)).max()) <= 1.2e-7
+    # both sides evaluate the same correctly rounded atanf: bit-identical
+    assert np.array_equal(out, oracle.heaviside(phi))
     rsf_amd.check(rsf_amd.lib().rsf_deri_heaviside_cal(_p(out), _p(phi), *_dims()))
     assert np.array_equal(out, oracle.deri_heaviside(phi))
 

@@ -300,3 +300,26 @@ def test_oracle_adjoint_matches_central_differences(sigma2):
 def _pdict(p):
     return dict(sigma=p.sigma, lambda1=p.lambda1, lambda2=p.lambda2, nu=p.nu, mu=p.mu, dt=p.dt,
                 epsilon=p.epsilon, sigma2=p.sigma2)
+
+
+def test_oracle_atan_matches_numpy():
+    """The oracle's float atanf comes from a double arctangent built from IEEE
+    basic operations; pinned against numpy's arctan in double (relative error)
+    and in float (the rounded results agree)."""
+    rng = np.random.default_rng(11)
+    x = np.concatenate([
+        np.linspace(-4, 4, 200001),
+        rng.uniform(-1, 1, 200000),
+        np.sign(rng.normal(size=200000)) * 10.0 ** rng.uniform(-12, 12, 200000),
+        [0.0, -0.0, 0.66, -0.66, 2.41421356237309504880, np.inf, -np.inf, 1e-300, 1e300],
+    ])
+    got = oracle.atan_f64(x)
+    ref = np.arctan(x)
+    nz = ref != 0
+    rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
+    assert rel.max() < 4e-16, rel.max()
+    assert np.all(got[~nz] == 0)
+    xf = x[np.abs(x) < 1e30].astype(np.float32)
+    gotf = oracle.atan_f64(xf.astype(np.float64)).astype(np.float32)
+    reff = np.arctan(xf.astype(np.float64)).astype(np.float32)
+    assert np.mean(gotf != reff) < 1e-5
