@@ -315,24 +315,37 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WP
 }
 
 // ---------------------------------------------------------------------------
-// Sweep 1 in two kernels (fast mode).  The fused kernel above keeps a z ring of
-// 2 x P accumulators per voxel in registers next to its LDS passes, which caps
-// its occupancy.  Here the x and y passes run alone (xy kernel, LDS only) and
-// write the two xy-blurred channels to a scratch volume; the z kernel marches
-// each (x, y) column through z with the register ring and emits F.  Same
-// operations in the same order as the fused kernel: identical results.
+// Sweep 1 in two kernels.  The fused kernel above keeps a z ring of CH x P
+// accumulators per voxel in registers next to its LDS passes, which caps its
+// occupancy.  Here the x and y passes run alone (xy kernel, LDS only) and write
+// the CH xy-blurred channels to a scratch volume; the z kernel marches each
+// (x, y) column through z with the register ring and emits F.  Same operations
+// in the same order as the fused kernel: identical results.  Channels: fast
+// mode I*G and G; literal mode I*H, H, I*(1-H), 1-H (main.cpp:238-246).
+
+// Channel values of one voxel from the transform t = T(phi) and the image i.
+template <bool LIT>
+__device__ __forceinline__ void sweep1_channels(float i, float t, float (&c)[LIT ? 4 : 2]) {
+    c[0] = i * t;                 // I*H (main.cpp:238) or I*G
+    c[1] = t;                     // H or G
+    if constexpr (LIT) {
+        const float hc = 1.0f - t;  // kernel.cu:295
+        c[2] = i * hc;              // I*(1-H) (main.cpp:246)
+        c[3] = hc;
+    }
+}
 
 // xy kernel: tiles of TX x TY outputs inside the valid region [M, w-M) x
-// [M, h-M) of a block, ZC planes per work item.  Per plane the (TY+2M) x
-// (TX+2M) window of I*G and G goes to LDS, the x pass writes (TY+2M) x TX
+// [M, h-M) of a block, zc planes per work item.  Per plane the (TY+2M) x
+// (TX+2M) window of the channels goes to LDS, the x pass writes (TY+2M) x TX
 // outputs to LDS (RX = 4 per work item), the y pass R outputs per work item
-// from R+P-1 LDS reads.  x-pass and y-pass work items are exact multiples of
-// the thread count.
-template <int P, int TX, int TY, int NT, int R, int WPE>
+// from R+P-1 LDS reads.  The work items of both passes are exact multiples of
+// the thread count.  Channel ch of the output goes to xy + ch * stride.
+template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_kernel(
-    const float* __restrict__ tr, const void* __restrict__ img, int img_u8, float* __restrict__ xyA,
-    float* __restrict__ xyB, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles,
-    int zc, Taps taps) {
+    const float* __restrict__ tr, const void* __restrict__ img, int img_u8, float* __restrict__ xy, int64_t stride,
+    const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc, Taps taps) {
+    constexpr int CH = LIT ? 4 : 2;
     constexpr int M = (P - 1) / 2;
     constexpr int RH = TY + 2 * M, RW = TX + 2 * M;
     constexpr int RWP = (RW + 3) & ~3;
@@ -341,19 +354,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     constexpr int NQ = RH * QW;
     constexpr int PF = (NQ + NT - 1) / NT;
     constexpr int RX = 4, XG = TX / RX;
-    constexpr int NXI = 2 * RH * XG;       // x-pass work items
+    constexpr int NXI = CH * RH * XG;      // x-pass work items
+    constexpr int NYI = CH * TX * (TY / R);  // y-pass work items
     constexpr int WIN = (RX + P - 1 + 3) & ~3;
-    static_assert(NXI % NT == 0, "x-pass items must be a multiple of the threads");
-    static_assert(2 * TX * (TY / R) == NT && TY % R == 0, "one y-pass item per thread");
-    __shared__ __align__(16) float sIn[2][RH][RWP];
-    __shared__ __align__(16) float sX[2][RH][TXP];
+    static_assert(NXI % NT == 0 && NYI % NT == 0 && TY % R == 0, "work items must be multiples of the threads");
+    __shared__ __align__(16) float sIn[CH][RH][RWP];
+    __shared__ __align__(16) float sX[CH][RH][TXP];
     const int tid = threadIdx.x;
     const uint8_t* img8 = (const uint8_t*)img;
     const float* imgf = (const float*)img;
-    // y-pass item of this thread: channel, column, first of R rows
-    const int ych = tid / (TX * (TY / R));
-    const int yrem = tid - ych * (TX * (TY / R));
-    const int yx = yrem % TX, yr0 = (yrem / TX) * R;
 
     for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
         const Tile tl = tiles[item];
@@ -414,10 +423,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 } else {
                     iv[0] = pi[k].x; iv[1] = pi[k].y; iv[2] = pi[k].z; iv[3] = pi[k].w;
                 }
-                // I*G and G
-                *reinterpret_cast<float4*>(&sIn[0][r][4 * q]) =
-                    make_float4(iv[0] * pt[k].x, iv[1] * pt[k].y, iv[2] * pt[k].z, iv[3] * pt[k].w);
-                *reinterpret_cast<float4*>(&sIn[1][r][4 * q]) = pt[k];
+                const float tv[4] = {pt[k].x, pt[k].y, pt[k].z, pt[k].w};
+                float cv[4][CH];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sweep1_channels<LIT>(iv[j], tv[j], cv[j]);
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch)
+                    *reinterpret_cast<float4*>(&sIn[ch][r][4 * q]) = make_float4(cv[0][ch], cv[1][ch], cv[2][ch], cv[3][ch]);
             }
         };
         fetch(z0);
@@ -454,187 +466,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 *reinterpret_cast<float4*>(&sX[ch][r][xb]) = make_float4(o4[0], o4[1], o4[2], o4[3]);
             }
             __syncthreads();
-            // ---- y pass: R outputs of column yx, rows yr0 .. yr0+R-1
-            {
-                float col[R + P - 1];
-#pragma unroll
-                for (int j = 0; j < R + P - 1; ++j) col[j] = sX[ych][yr0 + j][yx];
-                const int gx = x0 + yx;
-                float* dst = (ych == 0 ? xyA : xyB) + off + (int64_t)z * plane + gx;
-#pragma unroll
-                for (int o = 0; o < R; ++o) {
-                    float a = col[o] * taps.g[0];
-#pragma unroll
-                    for (int kk = 1; kk < P; ++kk) a = fmaf(col[o + kk], taps.g[kk], a);
-                    const int gy = y0 + yr0 + o;
-                    if (gx < w - M && gy < h - M) dst[(int64_t)gy * w] = a;
-                }
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// xy kernel, wave strips: as sweep1_xy_kernel, but each wave owns TX/NW
-// columns of the tile and runs the x pass and the y pass of its columns back
-// to back (its x-pass rows stay in a wave-private LDS strip), so only the
-// shared input window needs a workgroup barrier; that window is double
-// buffered, one barrier per plane.
-template <int P, int TX, int TY, int NW, int R, int WPE>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xyw_kernel(
-    const float* __restrict__ tr, const void* __restrict__ img, int img_u8, float* __restrict__ xyA,
-    float* __restrict__ xyB, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles,
-    int zc, Taps taps) {
-    constexpr int NT = 64 * NW;
-    constexpr int M = (P - 1) / 2;
-    constexpr int RH = TY + 2 * M, RW = TX + 2 * M;
-    constexpr int RWP = (RW + 3) & ~3;
-    constexpr int QW = RWP / 4;
-    constexpr int NQ = RH * QW;
-    constexpr int PF = (NQ + NT - 1) / NT;
-    constexpr int SW = TX / NW;            // strip width (columns per wave)
-    constexpr int SWP = SW + 4;            // padded strip row
-    constexpr int RX = 4, XG = SW / RX;
-    constexpr int NXI = 2 * RH * XG;       // x-pass items per wave
-    constexpr int WIN = (RX + P - 1 + 3) & ~3;
-    static_assert(SW % RX == 0 && NXI % 64 == 0, "x-pass items must fill the wave");
-    static_assert(2 * SW * (TY / R) == 64 && TY % R == 0, "one y-pass item per lane");
-    __shared__ __align__(16) float sIn[2][2][RH][RWP];
-    __shared__ __align__(16) float sX[NW][2][RH][SWP];
-    const int tid = threadIdx.x;
-    const int wv = tid / 64, lane = tid % 64;
-    const uint8_t* img8 = (const uint8_t*)img;
-    const float* imgf = (const float*)img;
-    // y-pass item of this lane: channel, strip column, first of R rows
-    const int ych = lane / (SW * (TY / R));
-    const int yrem = lane - ych * (SW * (TY / R));
-    const int yx = yrem % SW, yr0 = (yrem / SW) * R;
-
-    for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
-        const Tile tl = tiles[item];
-        if (tl.b < 0) continue;
-        const BlockDesc bd = blocks[tl.b];
-        const int w = bd.w, h = bd.h, l = bd.l;
-        const int64_t off = bd.off, plane = (int64_t)w * h;
-        const int x0 = tl.x0, y0 = tl.y0, z0 = tl.z0;
-        const int zend = min(z0 + zc, l);
-        const bool vec = ((off | (int64_t)w) & 3) == 0 && ((uintptr_t)tr & 15) == 0 &&
-                         (img_u8 ? ((uintptr_t)img8 & 3) == 0 : ((uintptr_t)imgf & 15) == 0);
-        float4 pt[PF], pi[PF];
-        auto fetch = [&](int z) {
-            const int64_t zoff = off + (int64_t)z * plane;
-#pragma unroll
-            for (int k = 0; k < PF; ++k) {
-                const int e = tid + k * NT;
-                if (NQ % NT != 0 && e >= NQ) break;
-                const int r = e / QW, q = e - r * QW;
-                const int gy = min(y0 - M + r, h - 1);
-                const int gx = x0 - M + 4 * q;
-                const int64_t row = zoff + (int64_t)gy * w;
-                if (vec && gx + 3 < w) {
-                    pt[k] = *reinterpret_cast<const float4*>(tr + row + gx);
-                    if (img_u8) {
-                        pi[k].x = __uint_as_float(*reinterpret_cast<const uint32_t*>(img8 + row + gx));
-                    } else {
-                        pi[k] = *reinterpret_cast<const float4*>(imgf + row + gx);
-                    }
-                } else {
-                    float tv[4], iv[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int64_t idx = row + min(gx + j, w - 1);
-                        tv[j] = tr[idx];
-                        iv[j] = img_u8 ? (float)img8[idx] : imgf[idx];
-                    }
-                    pt[k] = make_float4(tv[0], tv[1], tv[2], tv[3]);
-                    pi[k] = make_float4(iv[0], iv[1], iv[2], iv[3]);
-                    if (img_u8) pi[k].x = __uint_as_float((uint32_t)iv[0] | ((uint32_t)iv[1] << 8) |
-                                                          ((uint32_t)iv[2] << 16) | ((uint32_t)iv[3] << 24));
-                }
-            }
-        };
-        auto store = [&](int b) {
-#pragma unroll
-            for (int k = 0; k < PF; ++k) {
-                const int e = tid + k * NT;
-                if (NQ % NT != 0 && e >= NQ) break;
-                const int r = e / QW, q = e - r * QW;
-                float iv[4];
-                if (img_u8) {
-                    const uint32_t b4 = __float_as_uint(pi[k].x);
-                    iv[0] = (float)(b4 & 0xff);
-                    iv[1] = (float)((b4 >> 8) & 0xff);
-                    iv[2] = (float)((b4 >> 16) & 0xff);
-                    iv[3] = (float)(b4 >> 24);
-                } else {
-                    iv[0] = pi[k].x; iv[1] = pi[k].y; iv[2] = pi[k].z; iv[3] = pi[k].w;
-                }
-                *reinterpret_cast<float4*>(&sIn[b][0][r][4 * q]) =
-                    make_float4(iv[0] * pt[k].x, iv[1] * pt[k].y, iv[2] * pt[k].z, iv[3] * pt[k].w);
-                *reinterpret_cast<float4*>(&sIn[b][1][r][4 * q]) = pt[k];
-            }
-        };
-        fetch(z0);
-        for (int z = z0; z < zend; ++z) {
-            const int b = (z - z0) & 1;
-            store(b);
-            if (z + 1 < zend) fetch(z + 1);
-            // the input window of this plane is complete; the other buffer, read in
-            // the previous plane, is free for the next plane's store
-            __syncthreads();
-            // ---- x pass of this wave's strip (one item at a time)
+            // ---- y pass: R outputs of one column, rows r0 .. r0+R-1
 #pragma unroll 1
-            for (int k = 0; k < NXI / 64; ++k) {
-                const int e = lane + k * 64;
-                const int ch = e / (RH * XG);
-                const int rem = e - ch * (RH * XG);
-                const int r = rem / XG;
-                const int xs = (rem - r * XG) * RX;       // strip column
-                float win[WIN];
-                const float4* src = reinterpret_cast<const float4*>(&sIn[b][ch][r][wv * SW + xs]);
-#pragma unroll
-                for (int q = 0; q < WIN / 4; ++q) {
-                    const float4 v4 = src[q];
-                    win[4 * q + 0] = v4.x;
-                    win[4 * q + 1] = v4.y;
-                    win[4 * q + 2] = v4.z;
-                    win[4 * q + 3] = v4.w;
-                }
-                float o4[RX];
-#pragma unroll
-                for (int o = 0; o < RX; ++o) {
-                    float a = win[o] * taps.g[0];
-#pragma unroll
-                    for (int kk = 1; kk < P; ++kk) a = fmaf(win[o + kk], taps.g[kk], a);
-                    o4[o] = a;
-                }
-                *reinterpret_cast<float4*>(&sX[wv][ch][r][xs]) = make_float4(o4[0], o4[1], o4[2], o4[3]);
-            }
-            // the strip's x-pass rows are written by this wave alone: order the LDS
-            // writes before the reads below without a workgroup barrier
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // ---- y pass: R outputs of strip column yx, rows yr0 .. yr0+R-1
-            {
+            for (int k = 0; k < NYI / NT; ++k) {
+                const int e = tid + k * NT;
+                const int ch = e / (TX * (TY / R));
+                const int rem = e - ch * (TX * (TY / R));
+                const int cx = rem % TX, r0 = (rem / TX) * R;
                 float col[R + P - 1];
 #pragma unroll
-                for (int j = 0; j < R + P - 1; ++j) col[j] = sX[wv][ych][yr0 + j][yx];
-                const int gx = x0 + wv * SW + yx;
-                float* dst = (ych == 0 ? xyA : xyB) + off + (int64_t)z * plane + gx;
+                for (int j = 0; j < R + P - 1; ++j) col[j] = sX[ch][r0 + j][cx];
+                const int gx = x0 + cx;
+                float* dst = xy + ch * stride + off + (int64_t)z * plane + gx;
 #pragma unroll
                 for (int o = 0; o < R; ++o) {
                     float a = col[o] * taps.g[0];
 #pragma unroll
                     for (int kk = 1; kk < P; ++kk) a = fmaf(col[o + kk], taps.g[kk], a);
-                    const int gy = y0 + yr0 + o;
+                    const int gy = y0 + r0 + o;
                     if (gx < w - M && gy < h - M) dst[(int64_t)gy * w] = a;
                 }
             }
-            // the next x pass overwrites this wave's strip: finish the reads first
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         __syncthreads();
     }
@@ -647,12 +499,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
 // fits, 2 the fits and the blurred Heavisides.  All accesses are buffer
 // operations: the column's offset in a VGPR, the plane's offset in an SGPR
 // (blocks stay below 2^32 bytes per buffer).
-template <int P, int TXC, int TYC, int D, bool U8, int MODE>
+template <int P, bool LIT, int TXC, int TYC, int D, bool U8, int MODE>
 __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) void sweep1_z_kernel(
-    const float* __restrict__ xyA, const float* __restrict__ xyB, const void* __restrict__ img,
+    const float* __restrict__ xy, int64_t stride, const void* __restrict__ img,
     const float* __restrict__ blurI, float* __restrict__ out0, float* __restrict__ out1,
     float* __restrict__ out2, float* __restrict__ out3, const BlockDesc* __restrict__ blocks,
     const Tile* __restrict__ tiles, int ntiles, Taps taps, float halfS3, float l1, float l2, int same_lambda) {
+    constexpr int CH = LIT ? 4 : 2;
     constexpr int M = (P - 1) / 2;
     constexpr int ESZ = U8 ? 1 : 4;
     static_assert(P % D == 0, "P must be a multiple of the load queue depth");
@@ -667,7 +520,10 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         auto rsrc = [](const void* q) {
             return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(q), 0, (int)0xffffffff, 0x00020000);
         };
-        const __amdgpu_buffer_rsrc_t rA = rsrc(xyA + bd.off), rB = rsrc(xyB + bd.off), rL = rsrc(blurI + bd.off);
+        __amdgpu_buffer_rsrc_t rC[CH];
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) rC[ch] = rsrc(xy + ch * stride + bd.off);
+        const __amdgpu_buffer_rsrc_t rL = rsrc(LIT ? out0 + bd.off : blurI + bd.off);
         const __amdgpu_buffer_rsrc_t rI = rsrc((const char*)img + bd.off * ESZ);
         const __amdgpu_buffer_rsrc_t r0 = rsrc(out0 + bd.off), r1 = rsrc(out1 + bd.off);
         const __amdgpu_buffer_rsrc_t r2 = rsrc(out2 + bd.off), r3 = rsrc(out3 + bd.off);
@@ -683,9 +539,19 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         auto st = [&](__amdgpu_buffer_rsrc_t r, int z, float v) {
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, vc, 4 * z * plane, 0);
         };
-        auto store_out = [&](int z, float Iv, bool valid, float bI, float bIG, float bG) {
+        // ring outputs d[ch] (valid) or the pad band; bI: image blur (fast mode)
+        auto store_out = [&](int z, float Iv, bool valid, float bI, const float (&d)[CH]) {
             float fo = 1.0f, fi = 1.0f, bH = 30.0f, bHc = 30.0f;  // pad band (kernel.cu:313, main.cpp:277)
-            if (valid) fast_fits(bI, bIG, bG, halfS3, fo, fi, bH, bHc);
+            if (valid) {
+                if constexpr (LIT) {
+                    bH = d[1];
+                    bHc = d[3];
+                    fo = d[0] / bH;     // kernel.cu:203
+                    fi = d[2] / bHc;
+                } else {
+                    fast_fits(bI, d[0], d[1], halfS3, fo, fi, bH, bHc);
+                }
+            }
             if constexpr (MODE == 0) {
                 st(r0, z, force_of(Iv, fo, fi, l1, l2, same_lambda));
             } else {
@@ -697,20 +563,26 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
                 }
             }
         };
-        auto band = [&](int z) { store_out(z, ldi(z), false, 0.f, 0.f, 0.f); };
+        auto band = [&](int z) {
+            const float zero[CH] = {};
+            store_out(z, ldi(z), false, 0.f, zero);
+        };
         const bool xyvalid = x >= M && x < w - M && y >= M && y < h - M;
         if (!xyvalid) {
             for (int z = 0; z < l; ++z) band(z);
             continue;
         }
         for (int z = 0; z < M; ++z) band(z);
-        float acc0[P], acc1[P];
+        float acc[CH][P];
 #pragma unroll
-        for (int j = 0; j < P; ++j) acc0[j] = acc1[j] = 0.f;
-        // D planes of loads in flight: the xy channels of plane t and the image
-        // and image blur of the plane it completes, o = t - (P-1) + M.  The plane
-        // loop is unrolled by P (a multiple of D): ring and queue slots are fixed.
-        float qa[D], qb[D], qi[D], qB[D];
+        for (int ch = 0; ch < CH; ++ch)
+#pragma unroll
+            for (int j = 0; j < P; ++j) acc[ch][j] = 0.f;
+        // D planes of loads in flight: the channels of plane t and the image (and,
+        // fast mode, the image blur) of the plane it completes, o = t - (P-1) + M.
+        // The plane loop is unrolled by P (a multiple of D): ring and queue slots
+        // are fixed.
+        float qc[D][CH], qi[D], qB[D];
         auto group = [&](int t0, auto guard_) __attribute__((always_inline)) {
             constexpr bool GUARD = decltype(guard_)::value;
 #pragma unroll
@@ -718,14 +590,17 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
                 const int t = t0 + s;
                 if (GUARD && t >= l) break;
                 const int k = s % D;
-                const float va = qa[k], vb = qb[k], vi = qi[k], vB = qB[k];
+                float v[CH];
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) v[ch] = qc[k][ch];
+                const float vi = qi[k], vB = qB[k];
                 const int tn = t + D;
                 if (!GUARD || tn < l) {
-                    qa[k] = ldf(rA, tn);
-                    qb[k] = ldf(rB, tn);
+#pragma unroll
+                    for (int ch = 0; ch < CH; ++ch) qc[k][ch] = ldf(rC[ch], tn);
                     if (!GUARD || tn >= P - 1) {
                         qi[k] = ldi(tn - (P - 1) + M);
-                        qB[k] = ldf(rL, tn - (P - 1) + M);
+                        if constexpr (!LIT) qB[k] = ldf(rL, tn - (P - 1) + M);
                     }
                 }
                 // plane t adds tap g[(s - j) mod P] to accumulator j and completes
@@ -733,22 +608,25 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
                     const float g = taps.g[(s - j + P) % P];
-                    acc0[j] = fmaf(va, g, acc0[j]);
-                    acc1[j] = fmaf(vb, g, acc1[j]);
+#pragma unroll
+                    for (int ch = 0; ch < CH; ++ch) acc[ch][j] = fmaf(v[ch], g, acc[ch][j]);
                 }
                 const int js = (s + 1) % P;
-                const float d0 = acc0[js], d1 = acc1[js];
-                acc0[js] = 0.f;
-                acc1[js] = 0.f;
-                if (!GUARD || t >= P - 1) store_out(t - (P - 1) + M, vi, true, vB, d0, d1);
+                float d[CH];
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) {
+                    d[ch] = acc[ch][js];
+                    acc[ch][js] = 0.f;
+                }
+                if (!GUARD || t >= P - 1) store_out(t - (P - 1) + M, vi, true, vB, d);
             }
         };
         // queue fill: planes 0 .. D-1 (their completed planes exist only from P-1 on)
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (k < l) {
-                qa[k] = ldf(rA, k);
-                qb[k] = ldf(rB, k);
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) qc[k][ch] = ldf(rC[ch], k);
             }
         }
         group(0, std::true_type{});
@@ -1448,30 +1326,6 @@ Sweep1Variant s1v(const char* name) {
                          nullptr, 0, 0, {}};
 }
 // split sweep 1 (fast mode): xy kernel with TX x TY tiles, z kernel with TXC x TYC columns
-template <int P, int TX, int TY, int NW, int R, int WPE, int TXC, int TYC, int D>
-Sweep1Variant s1w(const char* name) {
-    Sweep1Variant v{P, 0, TX, TY, 64 * NW, (const void*)&sweep1_xyw_kernel<P, TX, TY, NW, R, WPE>, name,
-                    (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 0>, TXC, TYC, {}};
-    v.fn2s[0] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 0>;
-    v.fn2s[1] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 0>;
-    v.fn2s[2] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 1>;
-    v.fn2s[3] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 1>;
-    v.fn2s[4] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 2>;
-    v.fn2s[5] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 2>;
-    return v;
-}
-template <int P, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D>
-Sweep1Variant s1s(const char* name) {
-    Sweep1Variant v{P, 0, TX, TY, NT, (const void*)&sweep1_xy_kernel<P, TX, TY, NT, R, WPE>, name,
-                    (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 0>, TXC, TYC, {}};
-    v.fn2s[0] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 0>;
-    v.fn2s[1] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 0>;
-    v.fn2s[2] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 1>;
-    v.fn2s[3] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 1>;
-    v.fn2s[4] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, false, 2>;
-    v.fn2s[5] = (const void*)&sweep1_z_kernel<P, TXC, TYC, D, true, 2>;
-    return v;
-}
 template <bool LIT, int TX, int TY>
 Sweep2Variant s2v(const char* name) {
     return Sweep2Variant{LIT ? 1 : 0, TX, TY, 1, (const void*)&sweep2_kernel<LIT, TX, TY>, nullptr, nullptr, name};
@@ -1494,26 +1348,32 @@ Sweep2Variant s2q(const char* name) {
                          name};
 }
 
+// split sweep 1: xy kernel with TX x TY tiles, z kernel with TXC x TYC columns
+template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D>
+Sweep1Variant s1s(const char* name) {
+    Sweep1Variant v{P, LIT ? 1 : 0, TX, TY, NT, (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE>, name,
+                    (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0>, TXC, TYC, {}};
+    v.fn2s[0] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 0>;
+    v.fn2s[1] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0>;
+    v.fn2s[2] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 1>;
+    v.fn2s[3] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 1>;
+    v.fn2s[4] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 2>;
+    v.fn2s[5] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 2>;
+    return v;
+}
+
 }  // namespace
 
 const std::vector<Sweep1Variant>& sweep1_variants() {
     static const std::vector<Sweep1Variant> v = {
-        s1s<21, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
-        s1s<21, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3z256"),
-        s1s<21, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
-        s1s<21, 128, 28, 1024, 7, 4, 256, 1, 3>("p21_fast_split_128x28w4"),
-        s1s<21, 64, 28, 512, 7, 3, 256, 1, 3>("p21_fast_split_64x28w3"),
-        s1s<21, 32, 28, 256, 7, 3, 64, 4, 3>("p21_fast_split_32x28w3q3"),
+        s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
+        s1s<21, false, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
+        s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
+        s1s<21, true, 32, 28, 256, 7, 4, 256, 1, 3>("p21_lit_split_32x28w4"),
+        s1s<21, true, 64, 28, 512, 7, 4, 256, 1, 3>("p21_lit_split_64x28w4"),
         s1v<21, false, 32, 16, 512, 4>("p21_fast_32x16w4"),
-        s1v<21, false, 32, 16, 512, 4, true>("p21_fast_32x16w4db"),
-        s1v<21, false, 16, 20, 320, 4, true>("p21_fast_16x20w4db"),
-        s1v<21, false, 16, 20, 320, 4>("p21_fast_16x20w4"),
-        s1v<21, false, 32, 20, 640, 4, true>("p21_fast_32x20w4db"),
-        s1v<21, false, 32, 20, 640, 5, true>("p21_fast_32x20w5db"),
-        s1v<21, false, 16, 20, 320, 5, true>("p21_fast_16x20w5db"),
-        s1v<21, false, 32, 32, 1024, 4, true>("p21_fast_32x32t1kdb"),
-        s1v<21, false, 32, 16, 512>("p21_fast_32x16"),
         s1v<21, false, 32, 32, 1024, 4>("p21_fast_32x32t1k"),
+        s1v<21, false, 32, 32, 1024, 4, true>("p21_fast_32x32t1kdb"),
         s1v<21, true, 32, 16, 512>("p21_lit_32x16"),
         s1v<7, false, 32, 32, 512>("p7_fast_32x32"),
         s1v<7, true, 32, 16, 512>("p7_lit_32x16"),
@@ -1649,7 +1509,7 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     if (split1) {
         tp.grid1 = tp.tiles1.empty() ? 0 : grid_of(tp.tiles1.size(), 4);
         tp.grid5 = tp.tiles5.empty() ? 0 : grid_of(tp.tiles5.size(), 8);
-        RSF_TRY(tp.xy.ensure((size_t)tp.extent * 2 * sizeof(float)));
+        RSF_TRY(tp.xy.ensure((size_t)tp.extent * (V1[s1].literal ? 4 : 2) * sizeof(float)));
     }
     RSF_TRY(tp.d_tiles1.ensure(tp.tiles1.size() * sizeof(Tile)));
     RSF_TRY(tp.d_tiles2.ensure(tp.tiles2.size() * sizeof(Tile)));
@@ -1680,17 +1540,17 @@ int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, cons
     const int same = (p->lambda1 == p->lambda2) ? 1 : 0;
     if (v.fn2) {
         if (a.img_u8 != 0 && a.img_u8 != 1) return set_error(RSF_ERR_INVALID_ARGUMENT, "image dtype");
-        float* xyA = tp.xy.as<float>();
-        float* xyB = xyA + tp.extent;
+        float* xy = tp.xy.as<float>();
+        const int64_t stride = tp.extent;
         if (ntiles > 0) {
-            void* a1[] = {(void*)&a.tr, (void*)&a.img, (void*)&a.img_u8, (void*)&xyA, (void*)&xyB,
+            void* a1[] = {(void*)&a.tr, (void*)&a.img, (void*)&a.img_u8, (void*)&xy, (void*)&stride,
                           (void*)&tp.d_blocks.p, (void*)&tp.d_tiles1.p, (void*)&ntiles, (void*)&tp.zc1,
                           (void*)&d.taps};
             RSF_HIP(hipLaunchKernel(v.fn, dim3(tp.grid1), dim3(v.nt), a1, 0, s));
         }
         const int n5 = (int)tp.tiles5.size();
         if (a.out_fits < 0 || a.out_fits > 2) return set_error(RSF_ERR_INVALID_ARGUMENT, "sweep-1 output mode");
-        void* a2[] = {(void*)&xyA,      (void*)&xyB,      (void*)&a.img,    (void*)&a.blurI,
+        void* a2[] = {(void*)&xy,       (void*)&stride,   (void*)&a.img,    (void*)&a.blurI,
                       (void*)&a.out0,   (void*)&a.out1,   (void*)&a.out2,   (void*)&a.out3,
                       (void*)&tp.d_blocks.p, (void*)&tp.d_tiles5.p, (void*)&n5, (void*)&d.taps,
                       (void*)&d.halfS3, (void*)&p->lambda1, (void*)&p->lambda2, (void*)&same};
