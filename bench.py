@@ -6,9 +6,13 @@ phantom (300 tubes, radius 2-6, +-20 x ramp, noise sigma 15, seed 2), fp32 phi0
 = signed distance to the thresholded sigma-1 blur, sigma = 3 (21 taps),
 lambda1 = lambda2 = 1, nu = 65.025, mu = 1, dt = 0.1, epsilon = 1, 500
 iterations.  A step is one RSF iteration over the whole volume; `value` is
-voxel-iterations per second with image and phi resident in HBM.
+voxel-iterations per second with image and phi resident in HBM, in the literal
+mode (the reference's four blurred channels and operation order; bit-identical
+to the oracle).
 
 Also reported on the same line:
+  fast_mode     the same run in the fast mode (two blurred channels, H = 1/2 + G;
+                meets the one-step gate, not bit-identical);
   roofline      the dominant kernel's algorithmic bytes per launch over its
                 mean launch time (HIP events on the launch stream), against
                 MI355X's 8 TB/s; `traffic` = rocprofv3-measured bytes/launch
@@ -47,10 +51,15 @@ def params(rsf_amd, iterations, mode):
                                iterations=iterations, mode=mode)
 
 
-def plain_run(rsf_amd, torch, ctx, n, steps, warmup, mode, tune):
+def phantom(rsf_amd, n):
     img, _ = rsf_amd.synth_tubes((n, n, n), tubes=300, rmin=2.0, rmax=6.0, bias=20.0, noise=15.0, seed=2)
     phi0 = rsf_amd.initial_phi(img, 1.0, 110.0)
     log(f"generated {n}^3 phantom, {float((phi0 < 0).mean()):.3f} inside")
+    return img, phi0
+
+
+def plain_run(rsf_amd, torch, ctx, img, phi0, steps, warmup, mode, tune):
+    n = img.shape[0]
     dev = torch.device("cuda:0")
     d_img = torch.from_numpy(img).to(dev)
     d_phi0 = torch.from_numpy(phi0).to(dev)
@@ -73,7 +82,7 @@ def plain_run(rsf_amd, torch, ctx, n, steps, warmup, mode, tune):
     wall = time.perf_counter() - t0
     vox = float(n) ** 3
     inside = float((d_phi < 0).float().mean().item())
-    log(f"plain {n}^3 x {steps}: {wall * 1e3:.1f} ms wall, {loop_ms:.1f} ms loop")
+    log(f"plain {n}^3 x {steps} ({'literal' if mode else 'fast'}): {wall * 1e3:.1f} ms wall, {loop_ms:.1f} ms loop")
     # per-kernel timing, eager launches with events around each sweep
     d_phi.copy_(d_phi0)
     ms1, ms2 = ctx.profile_sweeps(blocks, d_img, d_phi, params(rsf_amd, 20, mode))
@@ -241,11 +250,12 @@ def cpu_baseline(img, phi0, budget_s):
     }
 
 
-def traffic_from_profiles(kernel):
+def traffic_from_profiles(kernel, mode):
+    """HBM bytes per launch of a sweep measured by rocprofv3 (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kernel, {}).get("bytes_per_launch")
+            return json.load(f).get(mode, {}).get(kernel, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
 
@@ -256,7 +266,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=512, help="cube side of the plain workload")
-    ap.add_argument("--mode", choices=["fast", "literal"], default="fast")
+    ap.add_argument("--mode", choices=["fast", "literal"], default="literal")
+    ap.add_argument("--no-fast", action="store_true", help="skip the fast-mode line of the literal run")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -295,13 +306,14 @@ def main():
     rsf_amd.lib()
     mode = rsf_amd.MODE_FAST if args.mode == "fast" else rsf_amd.MODE_LITERAL
     ctx = rsf_amd.Context(0)
-    r = plain_run(rsf_amd, torch, ctx, args.size, args.steps, args.warmup, mode, not args.no_tune)
+    img, phi0 = phantom(rsf_amd, args.size)
+    r = plain_run(rsf_amd, torch, ctx, img, phi0, args.steps, args.warmup, mode, not args.no_tune)
     N, K = r["vox"], args.steps
     value = N * K / r["wall"] / 1e9
     dom = "sweep1" if r["ms1"] >= r["ms2"] else "sweep2"
     ms_dom = r["ms1"] if dom == "sweep1" else r["ms2"]
     achieved = SWEEP_BYTES[dom] * N / (ms_dom * 1e-3) / 1e9
-    traffic = traffic_from_profiles(dom)
+    traffic = traffic_from_profiles(dom, args.mode)
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -317,7 +329,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"plain RSF drop-in: {args.size}^3 uint8 tube phantom (300 tubes, seed 2), fp32 phi, "
-                        f"sigma 3 (21 taps), lambda 1, nu 65.025, mu 1, dt 0.1, eps 1, {K} iterations",
+                        f"sigma 3 (21 taps), lambda 1, nu 65.025, mu 1, dt 0.1, eps 1, {K} iterations, "
+                        f"{args.mode} mode",
             "volume": [args.size] * 3,
             "iterations": K,
             "mode": args.mode,
@@ -343,6 +356,16 @@ def main():
     }
     if not args.no_batched:
         line["batched"] = batched_run(rsf_amd, torch, ctx, args.batched_steps, mode)
+    if mode == rsf_amd.MODE_LITERAL and not args.no_fast:
+        rf = plain_run(rsf_amd, torch, ctx, img, phi0, args.steps, args.warmup, rsf_amd.MODE_FAST, not args.no_tune)
+        line["fast_mode"] = {
+            "value": round(N * K / rf["wall"] / 1e9, 3),
+            "ms_per_step": round(rf["wall"] * 1e3 / K, 4),
+            "ms_per_launch": {"sweep1": round(rf["ms1"], 4), "sweep2": round(rf["ms2"], 4)},
+            "kernels": rf["choice"],
+            "parity": "one-step gate met (max|dphi| <= 2e-5 max|phi|); full-run sign masks within FP32 "
+                      "noise, not bit-identical",
+        }
     if args.adjoint:
         line["adjoint"] = adjoint_run(rsf_amd, torch, ctx)
     if args.large:
