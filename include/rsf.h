@@ -137,6 +137,44 @@ int rsf_profile_sweeps(rsf_context* ctx, const rsf_block* blocks, int nblocks, c
 int rsf_autotune(rsf_context* ctx, const rsf_block* blocks, int nblocks, int image_dtype,
                  const rsf_params* p, char* choice, int choice_len);
 
+/* ------------------------------------------------------- block pipeline
+ * The gigavoxel workflow of README.md:6-14,56-59: cut a volume into
+ * overlapping blocks, evolve every block, blend the blocks' phi back over the
+ * overlaps with linear ramps (interpolating_volumes.py:10-40, the reference's
+ * two halves along y generalised to every axis as a tensor product).
+ *
+ * Grid along each axis of n voxels with block size B and overlap k: B >= n
+ * gives one block; otherwise blocks start at j*(B-k) with size B, the last one
+ * clipped to n, and a last block shorter than max(min_size, 2k) is folded into
+ * the one before it.  B must exceed 2k.  Blocks are numbered x-fastest
+ * ((jz*Jy + jy)*Jx + jx) and packed back to back in that order.
+ * blocks/origins NULL = size query (nblocks, packed_voxels only); origins
+ * receives (x0, y0, z0) per block. */
+int rsf_block_grid(int64_t w, int64_t h, int64_t l, int64_t bw, int64_t bh, int64_t bl, int64_t overlap,
+                   int64_t min_size, rsf_block* blocks, int64_t* origins, int capacity, int* nblocks,
+                   int64_t* packed_voxels);
+/* Copies each block (origins[3i..3i+2], dims blocks[i].w/h/l) out of the
+ * X-fastest volume into packed + blocks[i].offset.  dtype: RSF_DTYPE_U8 or
+ * RSF_DTYPE_F32 (element size of both buffers).  Any block list that fits the
+ * volume is accepted (the blocks need not form a grid). */
+int rsf_disassemble_device(rsf_context* ctx, const void* volume, int dtype, int64_t w, int64_t h, int64_t l,
+                           const rsf_block* blocks, const int64_t* origins, int nblocks, void* packed);
+/* Inverse of rsf_disassemble_device for the grid rsf_block_grid(w..min_size)
+ * describes: volume[v] = sum over the blocks covering v of
+ * (wz*wy)*wx * phi_block, in double, rounded once to float.  Along an axis a
+ * voxel i voxels into the overlap of blocks j-1 and j has weights 1-alpha_i and
+ * alpha_i, alpha = numpy.linspace(0, 1, k); elsewhere weight 1.  For two
+ * blocks along y this is float32 of interpolate_volumes(A, B, k) bit for bit. */
+int rsf_reassemble_device(rsf_context* ctx, const float* packed, int64_t w, int64_t h, int64_t l, int64_t bw,
+                          int64_t bh, int64_t bl, int64_t overlap, int64_t min_size, float* volume);
+/* The whole pipeline on device buffers: grid (min_size = the tap count),
+ * disassemble image and phi, one batched run of p->iterations over all
+ * blocks, reassemble into phi (in place).  Replaces disassemble.py +
+ * rsfbatch.py + reassemble.py (README.md:56-59). */
+int rsf_run_volume_blocks_device(rsf_context* ctx, const void* image, int image_dtype, float* phi, int64_t w,
+                                 int64_t h, int64_t l, int64_t bw, int64_t bh, int64_t bl, int64_t overlap,
+                                 const rsf_params* p);
+
 /* ---------------------------------------------------------------- adjoint */
 /* Vector-Jacobian product of one step: given phi (state before the step) and
  * grad_out = dLoss/dphi', writes grad_in = dLoss/dphi and ADDS

@@ -563,6 +563,12 @@ int unfused_step(rsf_context* c, const float* I, const float* phi, float* out, i
 
 }  // namespace
 
+int rsf::context_stream(rsf_context* c, hipStream_t* s) {
+    RSF_TRY(check_ctx(c));
+    *s = c->stream;
+    return RSF_OK;
+}
+
 extern "C" {
 
 int rsf_abi_version(void) { return RSF_ABI_VERSION; }

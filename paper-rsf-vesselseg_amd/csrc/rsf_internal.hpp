@@ -28,6 +28,9 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
         if (rsf_s_ != RSF_OK) return rsf_s_; \
     } while (0)
 
+// Validates a context, makes its device current and returns its stream.
+int context_stream(rsf_context* c, hipStream_t* s);
+
 // Grow-only device allocation.
 struct DevBuf {
     void* p = nullptr;

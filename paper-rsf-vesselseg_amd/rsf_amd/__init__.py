@@ -101,6 +101,10 @@ _SIGS = {
     "rsf_profile_sweeps": (_I, [_P, _PB, _I, _P, _I, _P, _PP, C.POINTER(_F), C.POINTER(_F)]),
     "rsf_step_adjoint_device": (_I, [_P, _PB, _I, _P, _I, _P, _P, _P, _P, _PP]),
     "rsf_step_blocks_device": (_I, [_P, _PB, _I, _P, _I, _P, _P, _PP]),
+    "rsf_block_grid": (_I, [_I64] * 8 + [_PB, _P, _I, C.POINTER(_I), C.POINTER(_I64)]),
+    "rsf_disassemble_device": (_I, [_P, _P, _I, _I64, _I64, _I64, _PB, _P, _I, _P]),
+    "rsf_reassemble_device": (_I, [_P, _P] + [_I64] * 8 + [_P]),
+    "rsf_run_volume_blocks_device": (_I, [_P, _P, _I, _P] + [_I64] * 7 + [_PP]),
     "rsf_synth_tubes": (_I, [_P, _P, _I64, _I64, _I64, _I, _F, _F, _F, _F, C.c_uint64]),
     "rsf_initial_phi": (_I, [_P, _P, _I64, _I64, _I64, _F, _F]),
     "rsf_sdf_from_seed": (_I, [_P, _P, _I64, _I64, _I64, _F]),
@@ -313,6 +317,28 @@ class Context:
         check(lib().rsf_autotune(self._h, arr, len(blocks), int(image_dtype), C.byref(params), buf, 512))
         return buf.value.decode()
 
+    def disassemble(self, volume, grid, packed):
+        """Copy the blocks of `grid` (a BlockGrid) out of volume (l, h, w) into packed."""
+        l, h, w = volume.shape
+        _sync(volume, packed)
+        check(lib().rsf_disassemble_device(self._h, _ptr(volume), _dtype_code(volume), w, h, l, grid.c_blocks,
+                                           grid.c_origins.ctypes.data, len(grid.blocks), _ptr(packed)))
+
+    def reassemble(self, packed, grid, volume):
+        """Blend the packed blocks of `grid` into volume (l, h, w), float32."""
+        l, h, w = volume.shape
+        _sync(packed, volume)
+        check(lib().rsf_reassemble_device(self._h, _ptr(packed), w, h, l, *grid.block_size, grid.overlap,
+                                          grid.min_size, _ptr(volume)))
+
+    def run_volume_blocks(self, image, phi, block_size, overlap, params: Params):
+        """Whole block pipeline in place on phi (l, h, w): block_size = (bw, bh, bl)."""
+        l, h, w = phi.shape
+        bw, bh, bl = block_size
+        _sync(image, phi)
+        check(lib().rsf_run_volume_blocks_device(self._h, _ptr(image), _dtype_code(image), _ptr(phi), w, h, l,
+                                                 bw, bh, bl, overlap, C.byref(params)))
+
     def step_unfused(self, image_f32, phi_in, phi_out, params: Params):
         l, h, w = phi_in.shape
         _sync(image_f32, phi_in, phi_out)
@@ -326,6 +352,28 @@ def blocks_array(blocks):
     for k, (off, w, h, l) in enumerate(blocks):
         arr[k] = Block(int(off), int(w), int(h), int(l))
     return arr
+
+
+class BlockGrid:
+    """Overlapping block grid of a (l, h, w) volume, as rsf_block_grid lays it out.
+
+    blocks: [(offset, w, h, l)] in packed order; origins: (n, 3) int64 (x0, y0, z0)."""
+
+    def __init__(self, shape, block_size, overlap, min_size=1):
+        l, h, w = shape
+        self.shape = (int(l), int(h), int(w))
+        self.block_size = tuple(int(b) for b in block_size)  # (bw, bh, bl)
+        self.overlap, self.min_size = int(overlap), int(min_size)
+        n, tot = C.c_int(), C.c_int64()
+        args = (w, h, l, *self.block_size, self.overlap, self.min_size)
+        check(lib().rsf_block_grid(*args, None, None, 0, C.byref(n), C.byref(tot)))
+        self.c_blocks = (Block * n.value)()
+        self.c_origins = np.zeros((n.value, 3), np.int64)
+        check(lib().rsf_block_grid(*args, self.c_blocks, self.c_origins.ctypes.data, n.value, C.byref(n),
+                                   C.byref(tot)))
+        self.blocks = [(b.offset, b.w, b.h, b.l) for b in self.c_blocks]
+        self.origins = self.c_origins
+        self.packed_voxels = tot.value
 
 
 def pack_blocks(shapes):
