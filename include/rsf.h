@@ -174,6 +174,13 @@ int rsf_reassemble_device(rsf_context* ctx, const float* packed, int64_t w, int6
 int rsf_run_volume_blocks_device(rsf_context* ctx, const void* image, int image_dtype, float* phi, int64_t w,
                                  int64_t h, int64_t l, int64_t bw, int64_t bh, int64_t bl, int64_t overlap,
                                  const rsf_params* p);
+/* Host-pointer forms of rsf_run_batched_device and rsf_run_volume_blocks_device
+ * (used by the rsf CLI's --batch and --block modes). */
+int rsf_run_batched_host(rsf_context* ctx, const rsf_block* blocks, int nblocks, const void* image,
+                         int image_dtype, float* phi, const rsf_params* p);
+int rsf_run_volume_blocks_host(rsf_context* ctx, const void* image, int image_dtype, float* phi, int64_t w,
+                               int64_t h, int64_t l, int64_t bw, int64_t bh, int64_t bl, int64_t overlap,
+                               const rsf_params* p);
 
 /* ---------------------------------------------------------------- adjoint */
 /* Vector-Jacobian product of one step: given phi (state before the step) and
@@ -205,6 +212,12 @@ int rsf_initial_phi(const uint8_t* image, float* phi, int64_t w, int64_t h, int6
  * seed - shift > 0 form the region; phi is the signed Euclidean distance to
  * its boundary, POSITIVE inside that region (the sign of seed - shift). */
 int rsf_sdf_from_seed(const float* seed, float* phi, int64_t w, int64_t h, int64_t l, float shift);
+/* rsf_sdf_from_seed on device buffers (seed uint8 or float32, by seed_dtype),
+ * bit-identical to the host form: an exact separable squared EDT, x rows by
+ * forward/backward scans, y and z by outward scans.  The volume diagonal must
+ * be below 4096 voxels. */
+int rsf_sdf_from_seed_device(rsf_context* ctx, const void* seed, int seed_dtype, float* phi, int64_t w, int64_t h,
+                             int64_t l, float shift);
 
 /* ------------------------------------------ mirrors of the reference wrappers
  * Same names (with an rsf_ prefix), argument order and meaning as the host

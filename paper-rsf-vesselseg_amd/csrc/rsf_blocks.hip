@@ -274,4 +274,42 @@ int rsf_run_volume_blocks_device(rsf_context* c, const void* image, int image_dt
     return RSF_OK;
 }
 
+int rsf_run_batched_host(rsf_context* c, const rsf_block* blocks, int nblocks, const void* image, int image_dtype,
+                         float* phi, const rsf_params* p) {
+    hipStream_t s;
+    RSF_TRY(context_stream(c, &s));
+    if (!image || !phi || !blocks || nblocks <= 0) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL argument");
+    int64_t total = 0;
+    for (int i = 0; i < nblocks; ++i) total = std::max(total, blocks[i].offset + blocks[i].w * blocks[i].h * blocks[i].l);
+    const size_t ib = (size_t)total * (image_dtype == RSF_DTYPE_U8 ? 1 : 4);
+    DevBuf dimg, dphi;
+    RSF_TRY(dimg.ensure(ib));
+    RSF_TRY(dphi.ensure((size_t)total * sizeof(float)));
+    RSF_HIP(hipMemcpyAsync(dimg.p, image, ib, hipMemcpyHostToDevice, s));
+    RSF_HIP(hipMemcpyAsync(dphi.p, phi, (size_t)total * sizeof(float), hipMemcpyHostToDevice, s));
+    RSF_TRY(rsf_run_batched_device(c, blocks, nblocks, dimg.p, image_dtype, dphi.as<float>(), p));
+    RSF_HIP(hipMemcpyAsync(phi, dphi.p, (size_t)total * sizeof(float), hipMemcpyDeviceToHost, s));
+    RSF_HIP(hipStreamSynchronize(s));
+    return RSF_OK;
+}
+
+int rsf_run_volume_blocks_host(rsf_context* c, const void* image, int image_dtype, float* phi, int64_t w, int64_t h,
+                               int64_t l, int64_t bw, int64_t bh, int64_t bl, int64_t overlap, const rsf_params* p) {
+    hipStream_t s;
+    RSF_TRY(context_stream(c, &s));
+    if (!image || !phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (w <= 0 || h <= 0 || l <= 0) return set_error(RSF_ERR_INVALID_ARGUMENT, "dims must be positive");
+    const int64_t n = w * h * l;
+    const size_t ib = (size_t)n * (image_dtype == RSF_DTYPE_U8 ? 1 : 4);
+    DevBuf dimg, dphi;
+    RSF_TRY(dimg.ensure(ib));
+    RSF_TRY(dphi.ensure((size_t)n * sizeof(float)));
+    RSF_HIP(hipMemcpyAsync(dimg.p, image, ib, hipMemcpyHostToDevice, s));
+    RSF_HIP(hipMemcpyAsync(dphi.p, phi, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+    RSF_TRY(rsf_run_volume_blocks_device(c, dimg.p, image_dtype, dphi.as<float>(), w, h, l, bw, bh, bl, overlap, p));
+    RSF_HIP(hipMemcpyAsync(phi, dphi.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    RSF_HIP(hipStreamSynchronize(s));
+    return RSF_OK;
+}
+
 }  // extern "C"

@@ -7,11 +7,21 @@
 // the same exit code 1 for --help.  Additions: --output (alias of --ouput),
 // --lambda1/--lambda2 (default --e), --epsilon (default 0.2, the reference's
 // hard-coded value), --phi0 (take fp32 phi directly instead of sdf(seed-13)),
-// --mode fast|literal, --device.
+// --mode fast|literal, --device, and two block modes that replace the
+// reference's per-block process loop (README.md:56-59):
+//   --block N|W,H,L [--overlap K]  the volume is cut into overlapping blocks,
+//       all blocks evolve in one batched launch per sweep, and phi is blended
+//       back with linear ramps over the overlaps (interpolating_volumes.py);
+//   --batch manifest  one "image seed output" line per block file (paths
+//       relative to the manifest's folder): every block evolves in one batched
+//       launch and its phi is written to its output, as one rsf3d per line would.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -47,6 +57,9 @@ const Opt kOpts[] = {
     {"phi0", true, "initial level set (.npy, used instead of sdf(binary - 13))", nullptr},
     {"mode", true, "fast | literal (four blurred channels in reference order)", "fast"},
     {"device", true, "HIP device index", "0"},
+    {"block", true, "run as overlapping blocks of this size: N or W,H,L", nullptr},
+    {"overlap", true, "overlap of neighbouring blocks with --block", "32"},
+    {"batch", true, "manifest of 'image seed output' lines, run in one batched launch", nullptr},
 };
 constexpr int kNumOpts = sizeof(kOpts) / sizeof(kOpts[0]);
 
@@ -81,6 +94,134 @@ bool parse_float(const std::string& s, float* v) {
 int fail(const std::string& msg) {
     std::cerr << "rsf: " << msg << std::endl;
     return 1;
+}
+
+// Image as the library takes it: uint8 kept as is, anything else as fp32.
+struct Image {
+    rsf::NpyArray arr;
+    std::vector<float> f32;
+    bool u8 = false;
+};
+
+std::string load_image(const std::string& path, Image* im) {
+    std::string err = rsf::npy_load(path, &im->arr);
+    if (!err.empty()) return err;
+    if (im->arr.shape.size() != 3) return path + ": the image must be a 3-D array (Z, Y, X)";
+    im->u8 = im->arr.descr == "|u1" || im->arr.descr == "<u1";
+    if (!im->u8) return rsf::npy_to_f32(im->arr, &im->f32);
+    return "";
+}
+
+// phi0 = sdf(seed - 13) (main.cpp:174-183), or the array itself with --phi0.
+std::string load_phi0(const std::string& path, bool is_phi0, const std::vector<int64_t>& shape,
+                      std::vector<float>* phi) {
+    rsf::NpyArray s;
+    std::string err = rsf::npy_load(path, &s);
+    if (!err.empty()) return err;
+    if (s.shape != shape) return path + ": initial level set and image shapes differ";
+    std::vector<float> seed;
+    err = rsf::npy_to_f32(s, &seed);
+    if (!err.empty()) return err;
+    if (is_phi0) {
+        phi->swap(seed);
+        return "";
+    }
+    phi->resize(seed.size());
+    if (rsf_sdf_from_seed(seed.data(), phi->data(), shape[2], shape[1], shape[0], 13.0f) != RSF_OK)
+        return rsf_last_error();
+    return "";
+}
+
+void print_iterations(rsf_context* ctx, int T) {
+    float ms = 0.f;
+    rsf_last_loop_ms(ctx, &ms);
+    for (int k = 0; k < T; ++k) std::cout << "It takes " << ms / T << " ms to calculate " << std::endl;
+}
+
+bool parse_dims(const std::string& s, int64_t d[3]) {
+    std::vector<int64_t> v;
+    std::stringstream ss(s);
+    std::string tok;
+    while (std::getline(ss, tok, ',')) {
+        char* end = nullptr;
+        const long long x = std::strtoll(tok.c_str(), &end, 10);
+        if (tok.empty() || *end || x <= 0) return false;
+        v.push_back(x);
+    }
+    if (v.size() == 1) v = {v[0], v[0], v[0]};
+    if (v.size() != 3) return false;
+    std::copy(v.begin(), v.end(), d);
+    return true;
+}
+
+// --batch: every manifest line is one block (an rsf3d call of the reference's
+// batch driver); all blocks are packed back to back and run together.
+int run_batch(const std::string& manifest, bool is_phi0, const rsf_params& p, int device) {
+    std::ifstream f(manifest);
+    if (!f) return fail("cannot open manifest " + manifest);
+    const size_t slash = manifest.find_last_of('/');
+    const std::string dir = slash == std::string::npos ? "" : manifest.substr(0, slash + 1);
+    auto resolve = [&](const std::string& q) { return q.empty() || q[0] == '/' ? q : dir + q; };
+    struct Item {
+        Image im;
+        std::vector<float> phi;
+        std::string out;
+    };
+    std::vector<Item> items;
+    std::string line;
+    int lineno = 0;
+    while (std::getline(f, line)) {
+        ++lineno;
+        std::stringstream ss(line);
+        std::string a, b, c, extra;
+        if (!(ss >> a) || a[0] == '#') continue;
+        if (!(ss >> b >> c) || (ss >> extra))
+            return fail(manifest + ":" + std::to_string(lineno) + ": expected 'image seed output'");
+        items.emplace_back();
+        Item& it = items.back();
+        std::string err = load_image(resolve(a), &it.im);
+        if (err.empty()) err = load_phi0(resolve(b), is_phi0, it.im.arr.shape, &it.phi);
+        if (!err.empty()) return fail(err);
+        it.out = resolve(c);
+    }
+    if (items.empty()) return fail("manifest " + manifest + " lists no blocks");
+    const bool u8 = std::all_of(items.begin(), items.end(), [](const Item& it) { return it.im.u8; });
+    std::vector<rsf_block> blocks;
+    int64_t total = 0;
+    for (const Item& it : items) {
+        const auto& sh = it.im.arr.shape;
+        blocks.push_back(rsf_block{total, sh[2], sh[1], sh[0]});
+        total += sh[0] * sh[1] * sh[2];
+    }
+    std::vector<uint8_t> img8;
+    std::vector<float> imgf, phi((size_t)total);
+    if (u8) img8.resize((size_t)total);
+    else imgf.resize((size_t)total);
+    for (size_t i = 0; i < items.size(); ++i) {
+        const Item& it = items[i];
+        const size_t off = (size_t)blocks[i].offset, n = it.phi.size();
+        std::copy(it.phi.begin(), it.phi.end(), phi.begin() + off);
+        if (u8) {
+            std::copy(it.im.arr.data.begin(), it.im.arr.data.begin() + n, img8.begin() + off);
+        } else if (it.im.u8) {
+            for (size_t k = 0; k < n; ++k) imgf[off + k] = (float)it.im.arr.data[k];
+        } else {
+            std::copy(it.im.f32.begin(), it.im.f32.end(), imgf.begin() + off);
+        }
+    }
+    rsf_context* ctx = nullptr;
+    if (rsf_context_create(device, nullptr, &ctx) != RSF_OK) return fail(rsf_last_error());
+    const int st = rsf_run_batched_host(ctx, blocks.data(), (int)blocks.size(),
+                                        u8 ? (const void*)img8.data() : (const void*)imgf.data(),
+                                        u8 ? RSF_DTYPE_U8 : RSF_DTYPE_F32, phi.data(), &p);
+    if (st == RSF_OK) print_iterations(ctx, p.iterations);
+    rsf_context_destroy(ctx);
+    if (st != RSF_OK) return fail(std::string(rsf_status_string(st)) + ": " + rsf_last_error());
+    for (size_t i = 0; i < items.size(); ++i) {
+        const std::string err = rsf::npy_save_f32(items[i].out, phi.data() + blocks[i].offset, items[i].im.arr.shape);
+        if (!err.empty()) return fail(err);
+    }
+    return 0;
 }
 
 }  // namespace
@@ -140,48 +281,43 @@ int main(int argc, char** argv) {
     }
     int device = 0;
     if (seen[idx("device")]) device = std::atoi(val[idx("device")].c_str());
+    const bool use_phi0 = seen[idx("phi0")];
+    if (seen[idx("batch")]) {
+        if (seen[idx("image")] || seen[idx("binary")] || seen[idx("block")])
+            return fail("--batch takes its images and seeds from the manifest");
+        return run_batch(val[idx("batch")], use_phi0, p, device);
+    }
     std::string out_path = "out.npy";
     if (seen[idx("ouput")]) out_path = val[idx("ouput")];
     if (seen[idx("output")]) out_path = val[idx("output")];
     if (!seen[idx("image")]) return fail("--image is required");
     if (!seen[idx("binary")] && !seen[idx("phi0")]) return fail("--binary (or --phi0) is required");
 
-    rsf::NpyArray img;
-    std::string err = rsf::npy_load(val[idx("image")], &img);
+    Image im;
+    std::string err = load_image(val[idx("image")], &im);
     if (!err.empty()) return fail(err);
-    if (img.shape.size() != 3) return fail("--image must be a 3-D array (Z, Y, X)");
-    const int64_t l = img.shape[0], h = img.shape[1], w = img.shape[2];
-    const int64_t n = w * h * l;
-
+    const std::vector<int64_t>& shape = im.arr.shape;
+    const int64_t l = shape[0], h = shape[1], w = shape[2];
     std::vector<float> phi;
-    const bool use_phi0 = seen[idx("phi0")];
-    {
-        rsf::NpyArray s;
-        err = rsf::npy_load(val[idx(use_phi0 ? "phi0" : "binary")], &s);
-        if (!err.empty()) return fail(err);
-        if (s.shape != img.shape) return fail("initial level set and image shapes differ");
-        std::vector<float> seed;
-        err = rsf::npy_to_f32(s, &seed);
-        if (!err.empty()) return fail(err);
-        if (use_phi0) {
-            phi.swap(seed);
-        } else {
-            phi.resize((size_t)n);
-            if (rsf_sdf_from_seed(seed.data(), phi.data(), w, h, l, 13.0f) != RSF_OK)  // main.cpp:174-183
-                return fail(rsf_last_error());
-        }
-    }
-    const void* image_ptr;
-    int dtype;
-    std::vector<float> imgf;
-    if (img.descr == "|u1" || img.descr == "<u1") {
-        image_ptr = img.data.data();
-        dtype = RSF_DTYPE_U8;
-    } else {
-        err = rsf::npy_to_f32(img, &imgf);
-        if (!err.empty()) return fail(err);
-        image_ptr = imgf.data();
-        dtype = RSF_DTYPE_F32;
+    err = load_phi0(val[idx(use_phi0 ? "phi0" : "binary")], use_phi0, shape, &phi);
+    if (!err.empty()) return fail(err);
+    const void* image_ptr = im.u8 ? (const void*)im.arr.data.data() : (const void*)im.f32.data();
+    const int dtype = im.u8 ? RSF_DTYPE_U8 : RSF_DTYPE_F32;
+    if (seen[idx("block")]) {
+        int64_t bs[3];
+        if (!parse_dims(val[idx("block")], bs)) return fail("--block must be N or W,H,L (positive integers)");
+        float ov = 32.f;
+        if (seen[idx("overlap")] && (!parse_float(val[idx("overlap")], &ov) || ov < 0 || ov != (float)(int)ov))
+            return fail("--overlap must be a non-negative integer");
+        rsf_context* ctx = nullptr;
+        if (rsf_context_create(device, nullptr, &ctx) != RSF_OK) return fail(rsf_last_error());
+        const int st = rsf_run_volume_blocks_host(ctx, image_ptr, dtype, phi.data(), w, h, l, bs[0], bs[1], bs[2],
+                                                  (int64_t)ov, &p);
+        if (st == RSF_OK) print_iterations(ctx, p.iterations);
+        rsf_context_destroy(ctx);
+        if (st != RSF_OK) return fail(std::string(rsf_status_string(st)) + ": " + rsf_last_error());
+        err = rsf::npy_save_f32(out_path, phi.data(), shape);
+        return err.empty() ? 0 : fail(err);
     }
     rsf_context* ctx = nullptr;
     if (rsf_context_create(device, nullptr, &ctx) != RSF_OK) return fail(rsf_last_error());
@@ -190,7 +326,7 @@ int main(int argc, char** argv) {
     rsf_context_destroy(ctx);
     if (st != RSF_OK) return fail(std::string(rsf_status_string(st)) + ": " + rsf_last_error());
     for (int k = 0; k < p.iterations; ++k) std::cout << "It takes " << ms[k] << " ms to calculate " << std::endl;
-    err = rsf::npy_save_f32(out_path, phi.data(), img.shape);
+    err = rsf::npy_save_f32(out_path, phi.data(), shape);
     if (!err.empty()) return fail(err);
     return 0;
 }

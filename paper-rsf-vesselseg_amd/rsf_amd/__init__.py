@@ -105,9 +105,12 @@ _SIGS = {
     "rsf_disassemble_device": (_I, [_P, _P, _I, _I64, _I64, _I64, _PB, _P, _I, _P]),
     "rsf_reassemble_device": (_I, [_P, _P] + [_I64] * 8 + [_P]),
     "rsf_run_volume_blocks_device": (_I, [_P, _P, _I, _P] + [_I64] * 7 + [_PP]),
+    "rsf_run_batched_host": (_I, [_P, _PB, _I, _P, _I, _P, _PP]),
+    "rsf_run_volume_blocks_host": (_I, [_P, _P, _I, _P] + [_I64] * 7 + [_PP]),
     "rsf_synth_tubes": (_I, [_P, _P, _I64, _I64, _I64, _I, _F, _F, _F, _F, C.c_uint64]),
     "rsf_initial_phi": (_I, [_P, _P, _I64, _I64, _I64, _F, _F]),
     "rsf_sdf_from_seed": (_I, [_P, _P, _I64, _I64, _I64, _F]),
+    "rsf_sdf_from_seed_device": (_I, [_P, _P, _I, _P, _I64, _I64, _I64, _F]),
     "rsf_heaviside_cal": (_I, [_P, _P, _I, _I, _I]),
     "rsf_deri_heaviside_cal": (_I, [_P, _P, _I, _I, _I]),
     "rsf_inverse_cal": (_I, [_P, _P, _I, _I, _I]),
@@ -338,6 +341,12 @@ class Context:
         _sync(image, phi)
         check(lib().rsf_run_volume_blocks_device(self._h, _ptr(image), _dtype_code(image), _ptr(phi), w, h, l,
                                                  bw, bh, bl, overlap, C.byref(params)))
+
+    def sdf_from_seed(self, seed, phi, shift=13.0):
+        """phi0 = sdf(seed - shift) on the device (seed uint8 or float32, phi float32)."""
+        l, h, w = seed.shape
+        _sync(seed, phi)
+        check(lib().rsf_sdf_from_seed_device(self._h, _ptr(seed), _dtype_code(seed), _ptr(phi), w, h, l, shift))
 
     def step_unfused(self, image_f32, phi_in, phi_out, params: Params):
         l, h, w = phi_in.shape

@@ -170,3 +170,23 @@ def test_cli_input_errors(tmp_path):
     assert "--binary" in _cli("--image", str(img)).stderr
     assert _cli("--image", str(img), "--binary", str(img), "--t", "2.5").returncode != 0
     assert _cli("--image", str(img), "--binary", str(img), "--sigma", "abc").returncode != 0
+
+
+def test_cli_block_and_batch_argument_errors(tmp_path):
+    img = tmp_path / "img.npy"
+    np.save(img, np.zeros((24, 24, 24), np.uint8))
+    for bad in ("0", "32,32", "a", "32,32,32,32", "-4"):
+        r = _cli("--image", str(img), "--binary", str(img), "--block", bad)
+        assert r.returncode != 0 and "--block" in r.stderr, bad
+    r = _cli("--image", str(img), "--binary", str(img), "--block", "16", "--overlap", "2.5")
+    assert r.returncode != 0 and "--overlap" in r.stderr
+    assert "cannot open manifest" in _cli("--batch", str(tmp_path / "none.txt")).stderr
+    man = tmp_path / "m.txt"
+    man.write_text("# blocks\n\nimg.npy img.npy\n")
+    r = _cli("--batch", str(man))
+    assert r.returncode != 0 and "m.txt:3" in r.stderr
+    man.write_text("# nothing\n")
+    assert "lists no blocks" in _cli("--batch", str(man)).stderr
+    man.write_text("img.npy missing.npy out.npy\n")   # paths relative to the manifest
+    assert "missing.npy" in _cli("--batch", str(man)).stderr
+    assert _cli("--batch", str(man), "--image", str(img)).returncode != 0
