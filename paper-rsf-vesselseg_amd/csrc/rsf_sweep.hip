@@ -341,11 +341,16 @@ __device__ __forceinline__ void sweep1_channels(float i, float t, float (&c)[LIT
 // outputs to LDS (RX = 4 per work item), the y pass R outputs per work item
 // from R+P-1 LDS reads.  The work items of both passes are exact multiples of
 // the thread count.  Channel ch of the output goes to xy + ch * stride.
-template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE>
+template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, bool HI = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_kernel(
     const float* __restrict__ tr, const void* __restrict__ img, int img_u8, float* __restrict__ xy, int64_t stride,
     const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc, Taps taps) {
+    // HI (literal mode): LDS holds H and I, and each x-pass item forms a pair
+    // of channels from them -- (I*H, H) or (I*(1-H), 1-H) -- before blurring,
+    // which halves the LDS input tile.
+    static_assert(!HI || LIT, "HI is a literal-mode layout");
     constexpr int CH = LIT ? 4 : 2;
+    constexpr int CIN = HI ? 2 : CH;       // LDS input arrays
     constexpr int M = (P - 1) / 2;
     constexpr int RH = TY + 2 * M, RW = TX + 2 * M;
     constexpr int RWP = (RW + 3) & ~3;
@@ -354,11 +359,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     constexpr int NQ = RH * QW;
     constexpr int PF = (NQ + NT - 1) / NT;
     constexpr int RX = 4, XG = TX / RX;
-    constexpr int NXI = CH * RH * XG;      // x-pass work items
+    constexpr int NXI = (HI ? 2 : CH) * RH * XG;  // x-pass work items
     constexpr int NYI = CH * TX * (TY / R);  // y-pass work items
     constexpr int WIN = (RX + P - 1 + 3) & ~3;
     static_assert(NXI % NT == 0 && NYI % NT == 0 && TY % R == 0, "work items must be multiples of the threads");
-    __shared__ __align__(16) float sIn[CH][RH][RWP];
+    __shared__ __align__(16) float sIn[CIN][RH][RWP];
     __shared__ __align__(16) float sX[CH][RH][TXP];
     const int tid = threadIdx.x;
     const uint8_t* img8 = (const uint8_t*)img;
@@ -423,13 +428,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 } else {
                     iv[0] = pi[k].x; iv[1] = pi[k].y; iv[2] = pi[k].z; iv[3] = pi[k].w;
                 }
-                const float tv[4] = {pt[k].x, pt[k].y, pt[k].z, pt[k].w};
-                float cv[4][CH];
+                if constexpr (HI) {
+                    *reinterpret_cast<float4*>(&sIn[0][r][4 * q]) = pt[k];
+                    *reinterpret_cast<float4*>(&sIn[1][r][4 * q]) = make_float4(iv[0], iv[1], iv[2], iv[3]);
+                } else {
+                    const float tv[4] = {pt[k].x, pt[k].y, pt[k].z, pt[k].w};
+                    float cv[4][CH];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) sweep1_channels<LIT>(iv[j], tv[j], cv[j]);
+                    for (int j = 0; j < 4; ++j) sweep1_channels<LIT>(iv[j], tv[j], cv[j]);
 #pragma unroll
-                for (int ch = 0; ch < CH; ++ch)
-                    *reinterpret_cast<float4*>(&sIn[ch][r][4 * q]) = make_float4(cv[0][ch], cv[1][ch], cv[2][ch], cv[3][ch]);
+                    for (int ch = 0; ch < CH; ++ch)
+                        *reinterpret_cast<float4*>(&sIn[ch][r][4 * q]) =
+                            make_float4(cv[0][ch], cv[1][ch], cv[2][ch], cv[3][ch]);
+                }
             }
         };
         fetch(z0);
@@ -438,6 +449,45 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             if (z + 1 < zend) fetch(z + 1);
             __syncthreads();
             // ---- x pass (one item at a time: registers, not latency, are the limit)
+            if constexpr (HI) {
+#pragma unroll 1
+                for (int k = 0; k < NXI / NT; ++k) {
+                    const int e = tid + k * NT;
+                    const int pr = e / (RH * XG);
+                    const int rem = e - pr * (RH * XG);
+                    const int r = rem / XG;
+                    const int xb = (rem - r * XG) * RX;
+                    float wa[WIN], wb[WIN];
+                    const float4* sh = reinterpret_cast<const float4*>(&sIn[0][r][xb]);
+                    const float4* si = reinterpret_cast<const float4*>(&sIn[1][r][xb]);
+#pragma unroll
+                    for (int q = 0; q < WIN / 4; ++q) {
+                        const float4 h4 = sh[q], i4 = si[q];
+                        const float hv[4] = {h4.x, h4.y, h4.z, h4.w}, iv[4] = {i4.x, i4.y, i4.z, i4.w};
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            // the channel values of sweep1_channels: (I*H, H) or (I*(1-H), 1-H)
+                            const float t = pr == 0 ? hv[j] : 1.0f - hv[j];
+                            wa[4 * q + j] = iv[j] * t;
+                            wb[4 * q + j] = t;
+                        }
+                    }
+                    float oa[RX], ob[RX];
+#pragma unroll
+                    for (int o = 0; o < RX; ++o) {
+                        float a = wa[o] * taps.g[0], b = wb[o] * taps.g[0];
+#pragma unroll
+                        for (int kk = 1; kk < P; ++kk) {
+                            a = fmaf(wa[o + kk], taps.g[kk], a);
+                            b = fmaf(wb[o + kk], taps.g[kk], b);
+                        }
+                        oa[o] = a;
+                        ob[o] = b;
+                    }
+                    *reinterpret_cast<float4*>(&sX[2 * pr][r][xb]) = make_float4(oa[0], oa[1], oa[2], oa[3]);
+                    *reinterpret_cast<float4*>(&sX[2 * pr + 1][r][xb]) = make_float4(ob[0], ob[1], ob[2], ob[3]);
+                }
+            } else
 #pragma unroll 1
             for (int k = 0; k < NXI / NT; ++k) {
                 const int e = tid + k * NT;
@@ -1349,9 +1399,9 @@ Sweep2Variant s2q(const char* name) {
 }
 
 // split sweep 1: xy kernel with TX x TY tiles, z kernel with TXC x TYC columns
-template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D>
-Sweep1Variant s1s(const char* name) {
-    Sweep1Variant v{P, LIT ? 1 : 0, TX, TY, NT, (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE>, name,
+template <int P, bool LIT, int TX, int TY, int NT, int TXC, int TYC, int D>
+Sweep1Variant s1split(const void* xy_fn, const char* name) {
+    Sweep1Variant v{P, LIT ? 1 : 0, TX, TY, NT, xy_fn, name,
                     (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0>, TXC, TYC, {}};
     v.fn2s[0] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 0>;
     v.fn2s[1] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0>;
@@ -1362,6 +1412,11 @@ Sweep1Variant s1s(const char* name) {
     return v;
 }
 
+template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D, bool HI = false>
+Sweep1Variant s1s(const char* name) {
+    return s1split<P, LIT, TX, TY, NT, TXC, TYC, D>((const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI>,
+                                                   name);
+}
 }  // namespace
 
 const std::vector<Sweep1Variant>& sweep1_variants() {
@@ -1369,7 +1424,9 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, false, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
         s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
+        s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
         s1s<21, true, 32, 28, 256, 7, 4, 256, 1, 3>("p21_lit_split_32x28w4"),
+        s1s<21, true, 32, 28, 256, 7, 3, 256, 1, 3, true>("p21_lit_hi_32x28w3"),
         s1s<21, true, 64, 28, 512, 7, 4, 256, 1, 3>("p21_lit_split_64x28w4"),
         s1v<21, false, 32, 16, 512, 4>("p21_fast_32x16w4"),
         s1v<21, false, 32, 32, 1024, 4>("p21_fast_32x32t1k"),
