@@ -17,6 +17,8 @@ Also reported on the same line:
                 mean launch time (HIP events on the launch stream), against
                 MI355X's 8 TB/s; `traffic` = rocprofv3-measured bytes/launch
                 from profiles/ when present;
+                `valu` = sweep 1's minimum FP32 FMA count (the separable blurs)
+                over its launch time, against the 78.6 TFLOP/s VALU peak;
   loop_roofline 21 B per voxel-iteration over the whole timed loop;
   batched       64 uneven blocks (sides from {96,160,192,256}, seed 1000+b),
                 300 iterations: one persistent launch per sweep per
@@ -38,6 +40,19 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 METRIC = "Gvoxel-iterations/s + % HBM roofline; batched vs plain per-block, same GPU"
+# MI355X FP32 vector peak with one FMA per lane per cycle (packed FP32 is not
+# generated): 256 CUs x 64 lanes x 2 FLOP x 2.4 GHz.
+VALU_PEAK_TFLOPS = 78.6
+
+
+def valu_roofline(ms_sweep1, voxels, literal):
+    """Sweep 1's minimum arithmetic -- the separable 21-tap blur of each channel
+    (2 channels, 4 in literal mode) along x, y and z, 63 FMA = 126 FLOP per
+    channel-voxel -- over its mean launch time."""
+    flop = (4 if literal else 2) * 3 * 21 * 2
+    achieved = flop * voxels / (ms_sweep1 * 1e-3) / 1e12
+    return {"kernel": "sweep1", "flop_per_voxel": flop, "achieved": round(achieved, 2),
+            "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK_TFLOPS, 4)}
 HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES = {"sweep1": 9, "sweep2": 12}   # algorithmic bytes per voxel per launch
 
@@ -347,6 +362,9 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_voxel": SWEEP_BYTES[dom],
             "ms_per_launch": {"sweep1": round(r["ms1"], 4), "sweep2": round(r["ms2"], 4)},
+            # sweep 1 is bounded by FP32 FMA issue, not by HBM: its minimum arithmetic
+            # against the VALU peak without packed FP32
+            "valu": valu_roofline(r["ms1"], N, mode == rsf_amd.MODE_LITERAL),
         },
         "loop_roofline": {
             "bytes_per_voxel_iteration": 21,
@@ -362,6 +380,7 @@ def main():
             "value": round(N * K / rf["wall"] / 1e9, 3),
             "ms_per_step": round(rf["wall"] * 1e3 / K, 4),
             "ms_per_launch": {"sweep1": round(rf["ms1"], 4), "sweep2": round(rf["ms2"], 4)},
+            "valu": valu_roofline(rf["ms1"], N, False),
             "kernels": rf["choice"],
             "parity": "one-step gate met (max|dphi| <= 2e-5 max|phi|); full-run sign masks within FP32 "
                       "noise, not bit-identical",
