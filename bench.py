@@ -187,6 +187,55 @@ def large_run(rsf_amd, torch, ctx, steps, mode):
     return out
 
 
+def io_run(rsf_amd, torch, ctx):
+    """Streaming .npy I/O at the large-volume size (1024x1024x768): a uint8 image and an
+    fp32 phi file written to a temporary folder, then read into device memory by
+    rsf_npy_load_device and by numpy.load + a torch copy, and phi written back by
+    rsf_npy_save_device and by .cpu() + numpy.save.  The files sit in the page cache,
+    so this measures the host-side path, not the disk."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+
+    shape = (768, 1024, 1024)
+    n = int(np.prod(shape))
+    d = tempfile.mkdtemp()
+    out = {"volume": [shape[2], shape[1], shape[0]]}
+    try:
+        rng = np.random.default_rng(7)
+        np.save(os.path.join(d, "img.npy"), rng.integers(0, 256, shape, dtype=np.uint8))
+        np.save(os.path.join(d, "phi.npy"), rng.normal(size=shape).astype(np.float32))
+        for name, tdt, esz in (("img", torch.uint8, 1), ("phi", torch.float32, 4)):
+            path = os.path.join(d, name + ".npy")
+            dst = torch.empty(shape, dtype=tdt, device="cuda")
+            ctx.npy_load(path, dst)   # warm the page cache and the staging allocations
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.npy_load(path, dst)
+            torch.cuda.synchronize()
+            ts = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            ref = torch.from_numpy(np.load(path)).cuda()
+            torch.cuda.synchronize()
+            tn = time.perf_counter() - t0
+            out["load_" + name] = {"gib": round(n * esz / 2**30, 3), "streamed_gbs": round(n * esz / ts / 1e9, 2),
+                                   "numpy_gbs": round(n * esz / tn / 1e9, 2), "equal": bool(torch.equal(dst, ref))}
+            del dst, ref
+        src = torch.randn(shape, device="cuda")
+        t0 = time.perf_counter()
+        ctx.npy_save(os.path.join(d, "out.npy"), src)
+        ts = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        np.save(os.path.join(d, "out2.npy"), src.cpu().numpy())
+        tn = time.perf_counter() - t0
+        out["save_phi"] = {"streamed_gbs": round(4 * n / ts / 1e9, 2), "numpy_gbs": round(4 * n / tn / 1e9, 2)}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"io: {out}")
+    return out
+
+
 def adjoint_run(rsf_amd, torch, ctx, K=32, C=8, nvol=8, n=128):
     """Differentiated batch: nvol volumes of n^3 (seeds 4000+b), K unrolled steps with
     checkpoints every C, BCE of sigmoid(-phi/eps) against the generator's masks; forward
@@ -293,6 +342,8 @@ def main():
     ap.add_argument("--large-steps", type=int, default=1000)
     ap.add_argument("--adjoint", action="store_true",
                     help="also time the differentiated batch (8 x 128^3, 32 steps, checkpoints every 8)")
+    ap.add_argument("--io", action="store_true",
+                    help="also time streaming .npy load/save of a 1024x1024x768 volume against numpy")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scale for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -389,6 +440,8 @@ def main():
         line["adjoint"] = adjoint_run(rsf_amd, torch, ctx)
     if args.large:
         line["large"] = large_run(rsf_amd, torch, ctx, args.large_steps, mode)
+    if args.io:
+        line["io"] = io_run(rsf_amd, torch, ctx)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(r["img"], r["phi0"], args.cpu_budget)
     ctx.close()

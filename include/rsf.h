@@ -182,6 +182,26 @@ int rsf_run_volume_blocks_host(rsf_context* ctx, const void* image, int image_dt
                                int64_t h, int64_t l, int64_t bw, int64_t bh, int64_t bl, int64_t overlap,
                                const rsf_params* p);
 
+/* ------------------------------------------------------ streaming .npy I/O
+ * The reference reads and writes whole volumes through host memory
+ * (tira::volume load_npy / save_npy, main.cpp:145,149,316).  These stream a
+ * file in 32 MiB chunks through two pinned staging buffers, the file reads
+ * overlapping the PCIe copies on the context's stream. */
+/* Header of a .npy file: ndim, shape (up to 8 dims, C order) and whether it
+ * holds uint8/bool. */
+int rsf_npy_info(const char* path, int64_t* shape, int* ndim, int* is_u8);
+/* A 3-D .npy file into device memory: dst_dtype RSF_DTYPE_U8 copies a uint8
+ * file as is; RSF_DTYPE_F32 converts any supported dtype to float32.
+ * dst_bytes is the destination's capacity. */
+int rsf_npy_load_device(rsf_context* ctx, const char* path, void* dst, int dst_dtype, int64_t dst_bytes);
+/* A float32 (l, h, w) device volume into a .npy file. */
+int rsf_npy_save_device(rsf_context* ctx, const char* path, const float* src, int64_t w, int64_t h, int64_t l);
+/* The whole rsf3d run file to file (main.cpp:143-316): image and seed streamed
+ * in, phi0 = sdf(seed - 13) on the device (or the seed file taken as phi0),
+ * p->iterations steps, phi streamed out.  iter_ms as in rsf_run_device. */
+int rsf_run_npy(rsf_context* ctx, const char* image_path, const char* seed_path, int seed_is_phi0,
+                const char* out_path, const rsf_params* p, float* iter_ms);
+
 /* ---------------------------------------------------------------- adjoint */
 /* Vector-Jacobian product of one step: given phi (state before the step) and
  * grad_out = dLoss/dphi', writes grad_in = dLoss/dphi and ADDS

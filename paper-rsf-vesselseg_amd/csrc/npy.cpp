@@ -28,20 +28,19 @@ std::string dict_value(const std::string& hdr, const std::string& key) {
     return hdr.substr(p, e == std::string::npos ? std::string::npos : e - p);
 }
 
-size_t dtype_size(const std::string& d) {
+}  // namespace
+
+size_t npy_dtype_size(const std::string& d) {
     if (d.size() < 3) return 0;
     return (size_t)std::atoi(d.c_str() + 2);
 }
 
-}  // namespace
-
-std::string npy_load(const std::string& path, NpyArray* out) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f) return "cannot open '" + path + "'";
+std::string npy_read_header(std::istream& f, const std::string& path, NpyArray* out, int64_t* data_offset) {
     char magic[8];
     if (!f.read(magic, 8) || std::memcmp(magic, "\x93NUMPY", 6) != 0) return "'" + path + "' is not a .npy file";
     const int major = (unsigned char)magic[6];
     uint32_t hlen = 0;
+    int64_t pre = 10;
     if (major == 1) {
         unsigned char b[2];
         if (!f.read((char*)b, 2)) return "truncated .npy header in '" + path + "'";
@@ -50,6 +49,7 @@ std::string npy_load(const std::string& path, NpyArray* out) {
         unsigned char b[4];
         if (!f.read((char*)b, 4)) return "truncated .npy header in '" + path + "'";
         hlen = b[0] | (b[1] << 8) | (b[2] << 16) | ((uint32_t)b[3] << 24);
+        pre = 12;
     } else {
         return "unsupported .npy version " + std::to_string(major) + " in '" + path + "'";
     }
@@ -79,34 +79,48 @@ std::string npy_load(const std::string& path, NpyArray* out) {
         if (v < 0) return "negative dimension in '" + path + "'";
         out->shape.push_back(v);
     }
-    const size_t es = dtype_size(descr);
     const int64_t n = out->count();
     if (n < 0 || (uint64_t)n > (1ull << 40)) return "array too large in '" + path + "'";
+    *data_offset = pre + (int64_t)hlen;
+    return "";
+}
+
+std::string npy_load(const std::string& path, NpyArray* out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return "cannot open '" + path + "'";
+    int64_t data_offset = 0;
+    std::string err = npy_read_header(f, path, out, &data_offset);
+    if (!err.empty()) return err;
+    const size_t es = npy_dtype_size(out->descr);
+    const int64_t n = out->count();
     out->data.resize((size_t)n * es);
     if (n > 0 && !f.read((char*)out->data.data(), (std::streamsize)out->data.size()))
         return "truncated .npy payload in '" + path + "' (expected " + std::to_string(out->data.size()) + " bytes)";
     return "";
 }
 
-std::string npy_save_f32(const std::string& path, const float* data, const std::vector<int64_t>& shape) {
+std::string npy_header_f32(const std::vector<int64_t>& shape) {
     std::ostringstream h;
     h << "{'descr': '<f4', 'fortran_order': False, 'shape': (";
-    int64_t n = 1;
-    for (size_t k = 0; k < shape.size(); ++k) {
-        h << shape[k] << (shape.size() == 1 || k + 1 < shape.size() ? ", " : "");
-        n *= shape[k];
-    }
+    for (size_t k = 0; k < shape.size(); ++k) h << shape[k] << (shape.size() == 1 || k + 1 < shape.size() ? ", " : "");
     h << "), }";
     std::string hs = h.str();
     const size_t total = 10 + hs.size() + 1;
     hs += std::string((64 - total % 64) % 64, ' ');
     hs += '\n';
+    const uint16_t hl = (uint16_t)hs.size();
+    std::string out("\x93NUMPY\x01\x00", 8);
+    out += (char)(hl & 0xff);
+    out += (char)(hl >> 8);
+    return out + hs;
+}
+
+std::string npy_save_f32(const std::string& path, const float* data, const std::vector<int64_t>& shape) {
+    int64_t n = 1;
+    for (auto v : shape) n *= v;
+    const std::string hs = npy_header_f32(shape);
     std::ofstream f(path, std::ios::binary);
     if (!f) return "cannot write '" + path + "'";
-    f.write("\x93NUMPY\x01\x00", 8);
-    const uint16_t hl = (uint16_t)hs.size();
-    const unsigned char b[2] = {(unsigned char)(hl & 0xff), (unsigned char)(hl >> 8)};
-    f.write((const char*)b, 2);
     f.write(hs.data(), (std::streamsize)hs.size());
     f.write((const char*)data, (std::streamsize)(n * sizeof(float)));
     if (!f) return "write failed for '" + path + "'";
@@ -114,10 +128,11 @@ std::string npy_save_f32(const std::string& path, const float* data, const std::
 }
 
 std::string npy_to_f32(const NpyArray& a, std::vector<float>* out) {
-    const int64_t n = a.count();
-    out->resize((size_t)n);
-    const uint8_t* p = a.data.data();
-    const std::string& d = a.descr;
+    out->resize((size_t)a.count());
+    return npy_convert_f32(a.descr, a.data.data(), a.count(), out->data());
+}
+
+std::string npy_convert_f32(const std::string& d, const uint8_t* p, int64_t n, float* out) {
     for (int64_t i = 0; i < n; ++i) {
         float v;
         if (d == "|u1" || d == "<u1" || d == "|b1") v = (float)p[i];
@@ -129,7 +144,7 @@ std::string npy_to_f32(const NpyArray& a, std::vector<float>* out) {
         else if (d == "<f4") { std::memcpy(&v, p + 4 * i, 4); }
         else if (d == "<f8") { double x; std::memcpy(&x, p + 8 * i, 8); v = (float)x; }
         else return "unsupported dtype '" + d + "'";
-        (*out)[(size_t)i] = v;
+        out[i] = v;
     }
     return "";
 }

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <istream>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,14 @@ struct NpyArray {
 
 // Returns an empty string on success, else a message.
 std::string npy_load(const std::string& path, NpyArray* out);
+// Parses the header of an open .npy stream (descr and shape of *out; data left
+// empty) and gives the byte offset of the payload.
+std::string npy_read_header(std::istream& f, const std::string& path, NpyArray* out, int64_t* data_offset);
+size_t npy_dtype_size(const std::string& descr);
+// The bytes of a version-1.0 header for a C-ordered little-endian fp32 array.
+std::string npy_header_f32(const std::vector<int64_t>& shape);
+// Converts n elements of dtype descr at p to float32.
+std::string npy_convert_f32(const std::string& descr, const uint8_t* p, int64_t n, float* out);
 std::string npy_save_f32(const std::string& path, const float* data, const std::vector<int64_t>& shape);
 // Converts any supported numeric dtype to float32.
 std::string npy_to_f32(const NpyArray& a, std::vector<float>* out);

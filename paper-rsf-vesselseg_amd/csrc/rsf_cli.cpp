@@ -293,6 +293,26 @@ int main(int argc, char** argv) {
     if (!seen[idx("image")]) return fail("--image is required");
     if (!seen[idx("binary")] && !seen[idx("phi0")]) return fail("--binary (or --phi0) is required");
 
+    if (!seen[idx("block")]) {
+        // plain run, file to file on the device: image and seed streamed in, phi0 on the
+        // device, phi streamed out; the headers are checked before a device is touched
+        const std::string ip = val[idx("image")], sp = val[idx(use_phi0 ? "phi0" : "binary")];
+        int64_t is[8], ss[8];
+        int ind = 0, snd = 0, u8 = 0;
+        if (rsf_npy_info(ip.c_str(), is, &ind, &u8) != RSF_OK) return fail(rsf_last_error());
+        if (ind != 3) return fail("--image must be a 3-D array (Z, Y, X)");
+        if (rsf_npy_info(sp.c_str(), ss, &snd, &u8) != RSF_OK) return fail(rsf_last_error());
+        if (snd != 3 || ss[0] != is[0] || ss[1] != is[1] || ss[2] != is[2])
+            return fail("initial level set and image shapes differ");
+        rsf_context* ctx = nullptr;
+        if (rsf_context_create(device, nullptr, &ctx) != RSF_OK) return fail(rsf_last_error());
+        std::vector<float> ms((size_t)std::max(p.iterations, 1));
+        const int st = rsf_run_npy(ctx, ip.c_str(), sp.c_str(), use_phi0 ? 1 : 0, out_path.c_str(), &p, ms.data());
+        rsf_context_destroy(ctx);
+        if (st != RSF_OK) return fail(std::string(rsf_status_string(st)) + ": " + rsf_last_error());
+        for (int k = 0; k < p.iterations; ++k) std::cout << "It takes " << ms[k] << " ms to calculate " << std::endl;
+        return 0;
+    }
     Image im;
     std::string err = load_image(val[idx("image")], &im);
     if (!err.empty()) return fail(err);
@@ -319,14 +339,5 @@ int main(int argc, char** argv) {
         err = rsf::npy_save_f32(out_path, phi.data(), shape);
         return err.empty() ? 0 : fail(err);
     }
-    rsf_context* ctx = nullptr;
-    if (rsf_context_create(device, nullptr, &ctx) != RSF_OK) return fail(rsf_last_error());
-    std::vector<float> ms((size_t)std::max(p.iterations, 1));
-    int st = rsf_run_host(ctx, image_ptr, dtype, phi.data(), w, h, l, &p, ms.data());
-    rsf_context_destroy(ctx);
-    if (st != RSF_OK) return fail(std::string(rsf_status_string(st)) + ": " + rsf_last_error());
-    for (int k = 0; k < p.iterations; ++k) std::cout << "It takes " << ms[k] << " ms to calculate " << std::endl;
-    err = rsf::npy_save_f32(out_path, phi.data(), shape);
-    if (!err.empty()) return fail(err);
     return 0;
 }

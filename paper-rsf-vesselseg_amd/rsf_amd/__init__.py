@@ -107,6 +107,10 @@ _SIGS = {
     "rsf_run_volume_blocks_device": (_I, [_P, _P, _I, _P] + [_I64] * 7 + [_PP]),
     "rsf_run_batched_host": (_I, [_P, _PB, _I, _P, _I, _P, _PP]),
     "rsf_run_volume_blocks_host": (_I, [_P, _P, _I, _P] + [_I64] * 7 + [_PP]),
+    "rsf_npy_info": (_I, [C.c_char_p, C.POINTER(_I64), C.POINTER(_I), C.POINTER(_I)]),
+    "rsf_npy_load_device": (_I, [_P, C.c_char_p, _P, _I, _I64]),
+    "rsf_npy_save_device": (_I, [_P, C.c_char_p, _P, _I64, _I64, _I64]),
+    "rsf_run_npy": (_I, [_P, C.c_char_p, C.c_char_p, _I, C.c_char_p, _PP, _P]),
     "rsf_synth_tubes": (_I, [_P, _P, _I64, _I64, _I64, _I, _F, _F, _F, _F, C.c_uint64]),
     "rsf_initial_phi": (_I, [_P, _P, _I64, _I64, _I64, _F, _F]),
     "rsf_sdf_from_seed": (_I, [_P, _P, _I64, _I64, _I64, _F]),
@@ -348,6 +352,25 @@ class Context:
         _sync(seed, phi)
         check(lib().rsf_sdf_from_seed_device(self._h, _ptr(seed), _dtype_code(seed), _ptr(phi), w, h, l, shift))
 
+    def npy_load(self, path, dst):
+        """Stream a 3-D .npy file into the device tensor dst (uint8 as is, else float32)."""
+        _sync(dst)
+        check(lib().rsf_npy_load_device(self._h, os.fsencode(path), _ptr(dst), _dtype_code(dst),
+                                        dst.numel() * dst.element_size()))
+
+    def npy_save(self, path, src):
+        """Stream the float32 device volume src (l, h, w) into a .npy file."""
+        l, h, w = src.shape
+        _sync(src)
+        check(lib().rsf_npy_save_device(self._h, os.fsencode(path), _ptr(src), w, h, l))
+
+    def run_npy(self, image_path, seed_path, out_path, params: Params, seed_is_phi0=False, per_iteration=False):
+        """The rsf3d run file to file on the device; returns per-iteration ms if asked."""
+        ms = np.zeros(max(params.iterations, 1), np.float32) if per_iteration else None
+        check(lib().rsf_run_npy(self._h, os.fsencode(image_path), os.fsencode(seed_path), int(seed_is_phi0),
+                                os.fsencode(out_path), C.byref(params), _ptr(ms) if ms is not None else None))
+        return ms
+
     def step_unfused(self, image_f32, phi_in, phi_out, params: Params):
         l, h, w = phi_in.shape
         _sync(image_f32, phi_in, phi_out)
@@ -383,6 +406,14 @@ class BlockGrid:
         self.blocks = [(b.offset, b.w, b.h, b.l) for b in self.c_blocks]
         self.origins = self.c_origins
         self.packed_voxels = tot.value
+
+
+def npy_info(path):
+    """(shape, is_uint8) of a .npy file, from its header."""
+    shape = (C.c_int64 * 8)()
+    nd, u8 = C.c_int(), C.c_int()
+    check(lib().rsf_npy_info(os.fsencode(path), shape, C.byref(nd), C.byref(u8)))
+    return tuple(shape[k] for k in range(nd.value)), bool(u8.value)
 
 
 def pack_blocks(shapes):

@@ -190,3 +190,20 @@ def test_cli_block_and_batch_argument_errors(tmp_path):
     man.write_text("img.npy missing.npy out.npy\n")   # paths relative to the manifest
     assert "missing.npy" in _cli("--batch", str(man)).stderr
     assert _cli("--batch", str(man), "--image", str(img)).returncode != 0
+
+
+def test_npy_info_reads_headers(tmp_path):
+    cases = {"u8": np.zeros((3, 4, 5), np.uint8), "f4": np.zeros((2, 3, 4), np.float32),
+             "f8": np.zeros((6, 1, 2), np.float64), "i2": np.zeros((7,), np.int16), "b1": np.zeros((2, 2, 2), bool)}
+    for name, a in cases.items():
+        path = tmp_path / f"{name}.npy"
+        np.save(path, a)
+        shape, u8 = rsf_amd.npy_info(str(path))
+        assert shape == a.shape and u8 == (a.dtype in (np.uint8, np.bool_)), name
+    trunc = tmp_path / "trunc.npy"
+    _save_raw_npy(trunc, "{'descr': '<f4', 'fortran_order': False, 'shape': (4, 4, 4), }", b"\x00" * 12)
+    with pytest.raises(rsf_amd.RSFError, match="truncated"):
+        rsf_amd.npy_info(str(trunc))
+    (tmp_path / "bad.npy").write_bytes(b"nope")
+    with pytest.raises(rsf_amd.RSFError, match="not a .npy"):
+        rsf_amd.npy_info(str(tmp_path / "bad.npy"))
