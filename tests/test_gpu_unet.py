@@ -1,0 +1,87 @@
+"""The U-Net seed producer on the GPU: the vote bit-identical to the CPU one,
+the model on the GPU against the CPU, the seed hand-off into the device SDF
+and the RSF run, and gradients from an RSF loss back into the U-Net."""
+import numpy as np
+import pytest
+
+import rsf_amd
+from gpu_util import scene
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch  # noqa: F401
+
+    c = rsf_amd.Context(0)
+    yield c
+    c.close()
+
+
+def test_vote_on_device_equals_cpu():
+    import torch
+
+    from rsf_amd import unet
+
+    shape = (40, 100, 130)
+    sl = unet.tile_slices(shape)
+    pred = torch.rand((len(sl), unet.NUM_PICS, unet.HEIGHT, unet.WIDTH), generator=torch.Generator().manual_seed(1))
+    pc, mc = unet.vote(pred, shape, sl)
+    pg, mg = unet.vote(pred.cuda(), shape, sl)
+    assert torch.equal(pg.cpu(), pc) and torch.equal(mg.cpu(), mc)
+
+
+def test_segment_on_device_matches_cpu():
+    import torch
+
+    from rsf_amd import unet
+
+    torch.manual_seed(0)
+    model = unet.UNet3D().eval()
+    img, _, _ = scene((32, 96, 96), tubes=5, seed=61)
+    pc, _ = unet.segment(model, img, device="cpu", batch=8)
+    pg, _ = unet.segment(model.cuda(), img, device="cuda", batch=32)
+    assert float((pg.cpu() - pc).abs().max()) < 1e-4
+
+
+def test_seed_to_phi0_to_rsf(ctx):
+    import torch
+
+    from rsf_amd import unet
+
+    torch.manual_seed(0)
+    model = unet.UNet3D().cuda().eval()
+    img, _, _ = scene((32, 64, 64), tubes=4, seed=62)
+    prob, mask = unet.segment(model, img, device="cuda")
+    seed = unet.seed_volume(mask).contiguous()
+    phi = torch.empty(img.shape, device="cuda")
+    ctx.sdf_from_seed(seed, phi, 13.0)
+    ref = rsf_amd.sdf_from_seed(seed.cpu().numpy().astype(np.float32), 13.0)
+    assert np.array_equal(phi.cpu().numpy(), ref)
+    p = rsf_amd.make_params(sigma=1.0, epsilon=1.0, iterations=3)
+    ctx.run(torch.from_numpy(img).cuda(), phi, p)
+    assert bool(torch.isfinite(phi).all())
+
+
+def test_rsf_loss_backpropagates_into_the_unet():
+    import torch
+
+    from rsf_amd import unet
+    from rsf_amd.autograd import bce_inside_loss, rsf_unrolled
+
+    torch.manual_seed(0)
+    model = unet.UNet3D().cuda()
+    img, _, mask = scene((16, 64, 64), tubes=4, seed=63)
+    x = torch.from_numpy(unet.normalize(img)).float().cuda()[None, None]
+    prob = model(x)[0, 0]
+    phi0 = unet.phi0_from_probability(prob, scale=2.0).float().contiguous()
+    l, h, w = img.shape
+    out = rsf_unrolled(phi0.reshape(-1), torch.tensor([1.0, 1.0, 65.025, 1.0], device="cuda"),
+                       torch.from_numpy(img).cuda().reshape(-1), [(0, w, h, l)], K=4, checkpoint_every=2,
+                       sigma=1.0)
+    loss = bce_inside_loss(out, torch.from_numpy(mask).cuda().reshape(-1).float())
+    loss.backward()
+    grads = [p.grad for p in model.parameters()]
+    assert all(g is not None and bool(torch.isfinite(g).all()) for g in grads)
+    assert sum(float(g.abs().sum()) for g in grads) > 0
