@@ -236,6 +236,40 @@ def io_run(rsf_amd, torch, ctx):
     return out
 
 
+def unet_run(rsf_amd, torch, ctx):
+    """U-Net seed producer (rsf_amd.unet, random weights: the reference ships none) on a
+    128x512x512 tube phantom: 16x64x64 tiles at stride 32, fp32, 64 tiles per forward,
+    then the float64 vote; the host normalization is timed apart."""
+    from rsf_amd import unet
+
+    shape = (128, 512, 512)
+    img, _ = rsf_amd.synth_tubes(shape, tubes=80, seed=9)
+    torch.manual_seed(0)
+    model = unet.UNet3D().cuda().eval()
+    t0 = time.perf_counter()
+    x = torch.from_numpy(unet.normalize(img)).cuda()
+    torch.cuda.synchronize()
+    t_norm = time.perf_counter() - t0
+    sl = unet.tile_slices(shape)
+    unet.predict_tiles(model, x, sl[:64], 64)   # warm up the convolution kernels
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pred = unet.predict_tiles(model, x, sl, 64)
+    torch.cuda.synchronize()
+    t_pred = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    prob, mask = unet.vote(pred, shape, sl)
+    torch.cuda.synchronize()
+    t_vote = time.perf_counter() - t0
+    vox = float(shape[0] * shape[1] * shape[2])
+    out = {"volume": [shape[2], shape[1], shape[0]], "tiles": len(sl), "normalize_ms": round(t_norm * 1e3, 1),
+           "predict_ms": round(t_pred * 1e3, 1), "vote_ms": round(t_vote * 1e3, 1),
+           "mvox_per_s": round(vox / (t_pred + t_vote) / 1e6, 1),
+           "tiles_per_s": round(len(sl) / t_pred, 1), "inside_fraction": round(float(mask.double().mean()), 4)}
+    log(f"unet: {out}")
+    return out
+
+
 def adjoint_run(rsf_amd, torch, ctx, K=32, C=8, nvol=8, n=128):
     """Differentiated batch: nvol volumes of n^3 (seeds 4000+b), K unrolled steps with
     checkpoints every C, BCE of sigmoid(-phi/eps) against the generator's masks; forward
@@ -342,6 +376,8 @@ def main():
     ap.add_argument("--large-steps", type=int, default=1000)
     ap.add_argument("--adjoint", action="store_true",
                     help="also time the differentiated batch (8 x 128^3, 32 steps, checkpoints every 8)")
+    ap.add_argument("--unet", action="store_true",
+                    help="also time the U-Net seed producer's tiled inference on 128x512x512")
     ap.add_argument("--io", action="store_true",
                     help="also time streaming .npy load/save of a 1024x1024x768 volume against numpy")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -442,6 +478,8 @@ def main():
         line["large"] = large_run(rsf_amd, torch, ctx, args.large_steps, mode)
     if args.io:
         line["io"] = io_run(rsf_amd, torch, ctx)
+    if args.unet:
+        line["unet"] = unet_run(rsf_amd, torch, ctx)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(r["img"], r["phi0"], args.cpu_budget)
     ctx.close()
