@@ -1424,9 +1424,10 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, false, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
         s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
+        s1s<21, true, 32, 60, 256, 10, 2, 256, 1, 3, true>("p21_lit_hi_32x60r10"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
         s1s<21, true, 32, 28, 256, 7, 4, 256, 1, 3>("p21_lit_split_32x28w4"),
-        s1s<21, true, 32, 28, 256, 7, 3, 256, 1, 3, true>("p21_lit_hi_32x28w3"),
+        s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, true>("p21_lit_hi_32x60r15"),
         s1s<21, true, 64, 28, 512, 7, 4, 256, 1, 3>("p21_lit_split_64x28w4"),
         s1v<21, false, 32, 16, 512, 4>("p21_fast_32x16w4"),
         s1v<21, false, 32, 32, 1024, 4>("p21_fast_32x32t1k"),

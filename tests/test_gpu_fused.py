@@ -209,7 +209,8 @@ def test_unaligned_buffers_match_aligned(ctx, mode):
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.float32])
-def test_kernel_variants_give_identical_results(ctx, dtype, monkeypatch):
+@pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
+def test_kernel_variants_give_identical_results(ctx, dtype, mode, monkeypatch):
     """The fused sweep 1 and the split xy/z kernels, and every sweep-2 interior
     kernel (vector, scalar buffer, direct), compute the same operations in the
     same order: a run gives bit-identical phi whichever variants it takes."""
@@ -217,11 +218,17 @@ def test_kernel_variants_give_identical_results(ctx, dtype, monkeypatch):
 
     shape = (40, 44, 48)
     img, phi0, _ = scene(shape, tubes=6, seed=23)
-    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=3)
+    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=3, mode=mode)
     d_img = to_dev(img.astype(dtype))
     results = {}
-    for s1, s2 in [(None, None), ("p21_fast_32x16w4:64", None), (None, "fast_split_64x4:16"),
-                   (None, "fast_64x4:16"), ("p21_fast_32x16w4:16", "fast_vec_32x8:8")]:
+    if mode == rsf_amd.MODE_FAST:
+        combos = [(None, None), ("p21_fast_32x16w4:64", None), (None, "fast_split_64x4:16"),
+                  (None, "fast_64x4:16"), ("p21_fast_32x16w4:16", "fast_vec_32x8:8"),
+                  ("p21_fast_split_64x28w4:8", None)]
+    else:
+        combos = [(None, None), ("p21_lit_hi_32x44w2:16", None), ("p21_lit_split_32x28w4:64", "lit_split_64x4:16"),
+                  ("p21_lit_32x16:64", "lit_64x4:16"), ("p21_lit_hi_32x60r15:8", None)]
+    for s1, s2 in combos:
         for var, val in (("RSF_SWEEP1", s1), ("RSF_SWEEP2", s2)):
             if val is None:
                 monkeypatch.delenv(var, raising=False)
