@@ -37,18 +37,19 @@ struct Tile {
 // pi/2 - atan(1/x); x > 0.66: pi/4 + atan((x-1)/(x+1))).  Without contraction
 // every step rounds as on any IEEE machine, so the host oracle, which evaluates
 // the same sequence, gets the same bits.  Relative error below 2e-16.
+// The three reductions run as one division with selected operands, so a wave
+// whose lanes fall in different ranges takes no divergent branches: -1/ax is
+// -(1/ax) exactly, ax/1 is ax, and adding 0 to r >= 0 leaves it unchanged, so
+// the bits equal the branching form's.
 __device__ __forceinline__ double det_atan(double x) {
 #pragma clang fp contract(off)
     const double ax = x < 0 ? -x : x;
-    double y, v;
-    int flag;
-    if (ax > 2.41421356237309504880) {          // tan(3 pi / 8)
-        y = 1.57079632679489661923; flag = 1; v = -(1.0 / ax);
-    } else if (ax <= 0.66) {
-        y = 0.0; flag = 0; v = ax;
-    } else {
-        y = 7.85398163397448309616e-1; flag = 2; v = (ax - 1.0) / (ax + 1.0);
-    }
+    const bool big = ax > 2.41421356237309504880;  // tan(3 pi / 8)
+    const bool mid = !big && !(ax <= 0.66);
+    const double num = big ? -1.0 : (mid ? ax - 1.0 : ax);
+    const double den = big ? ax : (mid ? ax + 1.0 : 1.0);
+    const double v = num / den;
+    double y = big ? 1.57079632679489661923 : (mid ? 7.85398163397448309616e-1 : 0.0);
     const double z = v * v;
     const double pz = (((-8.750608600031904122785e-1 * z - 1.615753718733365076637e1) * z
                         - 7.500855792314704667340e1) * z - 1.228866684490136173410e2) * z
@@ -57,8 +58,7 @@ __device__ __forceinline__ double det_atan(double x) {
                         + 4.328810604912902668951e2) * z + 4.853903996359136964868e2) * z
                       + 1.945506571482613964425e2;
     double r = v * (z * pz / qz) + v;
-    if (flag == 2) r = r + 0.5 * 6.123233995736765886130e-17;
-    else if (flag == 1) r = r + 6.123233995736765886130e-17;
+    r = r + (big ? 6.123233995736765886130e-17 : (mid ? 0.5 * 6.123233995736765886130e-17 : 0.0));
     y = y + r;
     return x < 0 ? -y : y;
 }
