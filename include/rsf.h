@@ -92,6 +92,14 @@ int rsf_taps(float sigma, float* taps, int capacity, int* ntaps, int* pad);
 int rsf_context_create(int device, void* hip_stream, rsf_context** out);
 int rsf_context_destroy(rsf_context* ctx);
 int rsf_context_set_stream(rsf_context* ctx, void* hip_stream);
+/* The caller promises that the image at `image` does not change until
+ * rsf_context_release_image (or another hold).  Calls on this context with that
+ * image then reuse its preparation (float copy and, in FAST mode, its blur)
+ * while the block layout, taps and mode stay the same, instead of recomputing
+ * it on every call -- for loops of rsf_step_blocks_device /
+ * rsf_step_adjoint_device over one image.  No counterpart in the reference. */
+int rsf_context_hold_image(rsf_context* ctx, const void* image);
+int rsf_context_release_image(rsf_context* ctx);
 /* Bytes of device workspace a plain run of this size allocates. */
 int rsf_workspace_size(const rsf_params* p, int64_t w, int64_t h, int64_t l, int64_t* bytes);
 

@@ -179,13 +179,29 @@ __global__ __launch_bounds__(kT) void k_finish(float* gin, float* partials, cons
     if (threadIdx.x < 4) partials[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// grad_params[q] += sum over partial rows (fixed order: deterministic).
-__global__ void k_reduce_params(float* gparams, const float* partials, int rows) {
-    const int q = threadIdx.x;
-    if (q >= 4) return;
-    double s = 0.0;
-    for (int r = 0; r < rows; ++r) s += (double)partials[4 * r + q];
-    gparams[q] += (float)s;
+// grad_params[q] += sum over the partial rows, in double.  One workgroup: thread
+// t sums rows t, t + kR, t + 2 kR, ... and the kR sums meet in a fixed tree, so
+// the order (and the result) is the same on every call.
+constexpr int kR = 256;
+__global__ __launch_bounds__(kR) void k_reduce_params(float* gparams, const float* partials, int rows) {
+    __shared__ double red[4][kR];
+    const int t = threadIdx.x;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int r = t; r < rows; r += kR) {
+        const float4 v = *reinterpret_cast<const float4*>(partials + 4 * (int64_t)r);
+        s[0] += (double)v.x;
+        s[1] += (double)v.y;
+        s[2] += (double)v.z;
+        s[3] += (double)v.w;
+    }
+    for (int q = 0; q < 4; ++q) red[q][t] = s[q];
+    __syncthreads();
+    for (int o = kR / 2; o > 0; o >>= 1) {
+        if (t < o)
+            for (int q = 0; q < 4; ++q) red[q][t] += red[q][t + o];
+        __syncthreads();
+    }
+    if (t < 4) gparams[t] += (float)red[t][0];
 }
 
 }  // namespace
@@ -234,7 +250,7 @@ int finish(float* gin, float* partials, const float* phi, const float* gout, con
 }
 
 int reduce_params(float* gparams, const float* partials, int rows, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_params, dim3(1), dim3(64), 0, s, gparams, partials, rows);
+    hipLaunchKernelGGL(k_reduce_params, dim3(1), dim3(kR), 0, s, gparams, partials, rows);
     RSF_HIP(hipGetLastError());
     return RSF_OK;
 }

@@ -160,6 +160,10 @@ struct rsf_context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> iter_events;
     float last_ms = 0.f;
+    // rsf_context_hold_image: an image the caller keeps unchanged, and the key of
+    // the preparation (float copy, blur) last computed for it
+    const void* held_image = nullptr;
+    std::string held_key;
 };
 
 namespace {
@@ -412,11 +416,26 @@ int ensure_workspace(rsf_context* c, const Prepared& pr) {
 // Once per run: image as float and, in fast mode, the padded blur of the image.
 int prepare_image(rsf_context* c, const Prepared& pr, const void* image, int u8) {
     if (pr.literal && pr.sw == 0) return RSF_OK;
+    // a held image keeps its preparation while the layout, taps, mode and workspace match
+    std::string key;
+    if (image == c->held_image) {
+        std::ostringstream os;
+        os << blocks_key(pr.blocks) << '|' << u8 << '|' << pr.literal << '|' << pr.sw << '|' << c->ws_epoch << '|';
+        for (int k = 0; k < pr.d.P; ++k) {
+            uint32_t bits;
+            std::memcpy(&bits, &pr.d.taps.g[k], 4);
+            os << std::hex << bits << ',';
+        }
+        key = os.str();
+        if (key == c->held_key) return RSF_OK;
+        c->held_key.clear();
+    }
     RSF_TRY(ops::to_float(c->imgf.as<float>(), image, u8, pr.extent, c->stream));
-    if (pr.literal) return RSF_OK;
-    for (auto& b : pr.blocks)
-        RSF_TRY(ops::blur_pad(c->blurI.as<float>() + b.off, c->imgf.as<float>() + b.off, c->tmp1.as<float>(),
-                              c->tmp2.as<float>(), b.w, b.h, b.l, pr.d.taps, pr.d.P, pr.d.m, c->stream));
+    if (!pr.literal)
+        for (auto& b : pr.blocks)
+            RSF_TRY(ops::blur_pad(c->blurI.as<float>() + b.off, c->imgf.as<float>() + b.off, c->tmp1.as<float>(),
+                                  c->tmp2.as<float>(), b.w, b.h, b.l, pr.d.taps, pr.d.P, pr.d.m, c->stream));
+    if (!key.empty()) c->held_key = key;
     return RSF_OK;
 }
 
@@ -654,6 +673,20 @@ int rsf_context_set_stream(rsf_context* c, void* hip_stream) {
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     c->stream = s;
     c->own_stream = false;
+    return RSF_OK;
+}
+
+int rsf_context_hold_image(rsf_context* c, const void* image) {
+    RSF_TRY(check_ctx(c));
+    c->held_image = image;
+    c->held_key.clear();
+    return RSF_OK;
+}
+
+int rsf_context_release_image(rsf_context* c) {
+    RSF_TRY(check_ctx(c));
+    c->held_image = nullptr;
+    c->held_key.clear();
     return RSF_OK;
 }
 

@@ -90,6 +90,8 @@ _SIGS = {
     "rsf_context_create": (_I, [_I, _P, C.POINTER(_P)]),
     "rsf_context_destroy": (_I, [_P]),
     "rsf_context_set_stream": (_I, [_P, _P]),
+    "rsf_context_hold_image": (_I, [_P, _P]),
+    "rsf_context_release_image": (_I, [_P]),
     "rsf_workspace_size": (_I, [_PP, _I64, _I64, _I64, C.POINTER(_I64)]),
     "rsf_step_device": (_I, [_P, _P, _I, _P, _P, _I64, _I64, _I64, _PP]),
     "rsf_run_device": (_I, [_P, _P, _I, _P, _I64, _I64, _I64, _PP, _P]),
@@ -351,6 +353,15 @@ class Context:
         l, h, w = seed.shape
         _sync(seed, phi)
         check(lib().rsf_sdf_from_seed_device(self._h, _ptr(seed), _dtype_code(seed), _ptr(phi), w, h, l, shift))
+
+    def hold_image(self, image):
+        """Promise that `image` stays unchanged until release_image(): its float copy
+        and blur are then computed once instead of on every step."""
+        _sync(image)
+        check(lib().rsf_context_hold_image(self._h, _ptr(image)))
+
+    def release_image(self):
+        check(lib().rsf_context_release_image(self._h))
 
     def npy_load(self, path, dst):
         """Stream a 3-D .npy file into the device tensor dst (uint8 as is, else float32)."""

@@ -26,11 +26,15 @@ class _Unrolled(torch.autograd.Function):
         cur = phi0.detach().contiguous().clone()
         nxt = torch.empty_like(cur)
         ckpts = [cur.clone()]
-        for t in range(K):
-            rsf.step_blocks(blocks, image, cur, nxt, p)
-            cur, nxt = nxt, cur
-            if (t + 1) % checkpoint_every == 0 and t + 1 < K:
-                ckpts.append(cur.clone())
+        rsf.hold_image(image)   # the image is constant over the unrolled steps
+        try:
+            for t in range(K):
+                rsf.step_blocks(blocks, image, cur, nxt, p)
+                cur, nxt = nxt, cur
+                if (t + 1) % checkpoint_every == 0 and t + 1 < K:
+                    ckpts.append(cur.clone())
+        finally:
+            rsf.release_image()
         ctx.rsf, ctx.blocks, ctx.image, ctx.p = rsf, blocks, image, p
         ctx.K, ctx.C, ctx.ckpts = K, checkpoint_every, ckpts
         ctx.theta_device = theta.device
@@ -43,19 +47,23 @@ class _Unrolled(torch.autograd.Function):
         gp = torch.zeros(4, dtype=torch.float32, device=g.device)
         K, C = ctx.K, ctx.C
         starts = list(range(0, K, C))
-        for si in reversed(range(len(starts))):
-            s = starts[si]
-            e = min(s + C, K)
-            states = [ctx.ckpts[si]]
-            for _ in range(s, e - 1):
-                nxt = torch.empty_like(states[-1])
-                rsf.step_blocks(blocks, image, states[-1], nxt, p)
-                states.append(nxt)
-            for t in reversed(range(s, e)):
-                gin = torch.empty_like(g)
-                rsf.adjoint(blocks, image, states[t - s], g, gin, gp, p)
-                g = gin
-            del states
+        rsf.hold_image(image)
+        try:
+            for si in reversed(range(len(starts))):
+                s = starts[si]
+                e = min(s + C, K)
+                states = [ctx.ckpts[si]]
+                for _ in range(s, e - 1):
+                    nxt = torch.empty_like(states[-1])
+                    rsf.step_blocks(blocks, image, states[-1], nxt, p)
+                    states.append(nxt)
+                for t in reversed(range(s, e)):
+                    gin = torch.empty_like(g)
+                    rsf.adjoint(blocks, image, states[t - s], g, gin, gp, p)
+                    g = gin
+                del states
+        finally:
+            rsf.release_image()
         ctx.ckpts = None
         return g, gp.to(ctx.theta_device), None, None, None, None, None, None
 

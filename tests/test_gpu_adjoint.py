@@ -176,3 +176,58 @@ def test_unrolled_gradient_matches_fp64_adjoint_and_central_differences(ctx):
         # lambda moves phi by ~dt*delta*F ~ 1e3 per unit: even a 1e-6 step (the smallest the
         # float32 parameter holds) is not fully linear at the degenerate normals, so 1e-3 here
         assert abs(ref_th[k] - fd) <= 1e-3 * max(abs(fd), 1e-3 * float(np.abs(ref_th).max())), (name, ref_th[k], fd)
+
+
+def test_held_image_gives_identical_results(ctx):
+    """rsf_context_hold_image reuses the image's preparation across calls: the
+    steps and the adjoint give the same bits as without it, and after a release
+    a changed image is prepared afresh."""
+    import torch
+
+    shapes = [(30, 32, 34), (28, 30, 36)]
+    blocks, total = rsf_amd.pack_blocks(shapes)
+    img = np.empty(total, np.uint8)
+    phi = np.empty(total, np.float32)
+    for k, ((off, w, h, l), shp) in enumerate(zip(blocks, shapes)):
+        bi, bp, _ = scene(shp, tubes=4, seed=70 + k)
+        img[off:off + bi.size] = bi.ravel()
+        phi[off:off + bp.size] = bp.ravel()
+    p = _params(3.0)
+    d_img = to_dev(img)
+    u = to_dev(np.random.default_rng(8).normal(size=total).astype(np.float32))
+
+    def run(hold):
+        if hold:
+            ctx.hold_image(d_img)
+        try:
+            a, b = to_dev(phi), torch.empty(total, device="cuda")
+            for _ in range(3):
+                ctx.step_blocks(blocks, d_img, a, b, p)
+                a, b = b, a
+            gin = torch.empty(total, device="cuda")
+            gp = torch.zeros(4, device="cuda")
+            ctx.adjoint(blocks, d_img, a, u, gin, gp, p)
+            torch.cuda.synchronize()
+            return a.cpu().numpy(), gin.cpu().numpy(), gp.cpu().numpy()
+        finally:
+            if hold:
+                ctx.release_image()
+
+    ref = run(False)
+    got = run(True)
+    for r, g in zip(ref, got):
+        assert np.array_equal(r, g)
+    # released: an edited image is prepared again
+    ctx.hold_image(d_img)
+    run(False)
+    ctx.release_image()
+    d_img[:100] = 255
+    img2 = d_img.cpu().numpy()
+    fresh = rsf_amd.Context(0)
+    a1, b1 = to_dev(phi), torch.empty(total, device="cuda")
+    a2, b2 = to_dev(phi), torch.empty(total, device="cuda")
+    ctx.step_blocks(blocks, d_img, a1, b1, p)
+    fresh.step_blocks(blocks, to_dev(img2), a2, b2, p)
+    torch.cuda.synchronize()
+    fresh.close()
+    assert np.array_equal(b1.cpu().numpy(), b2.cpu().numpy())
