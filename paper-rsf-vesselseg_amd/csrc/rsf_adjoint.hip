@@ -34,6 +34,21 @@ inline dim3 grid_for(int64_t n) {
 #define VOXELS(i, n) \
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
+// x, y, z of a block-local linear index; 32-bit division when the block allows it
+__device__ __forceinline__ void coords(int64_t i, int w, int h, int64_t n, int& x, int& y, int& z) {
+    if (n < (int64_t(1) << 31)) {
+        const uint32_t ii = (uint32_t)i, r = ii / (uint32_t)w;
+        x = (int)(ii - r * (uint32_t)w);
+        z = (int)(r / (uint32_t)h);
+        y = (int)(r - (uint32_t)z * (uint32_t)h);
+    } else {
+        const int64_t r = i / w;
+        x = (int)(i - r * w);
+        z = (int)(r / h);
+        y = (int)(r - (int64_t)z * h);
+    }
+}
+
 __device__ __forceinline__ float dirac(float p, float eps) {
     return (float)(1.0 / kPi) * eps / (eps * eps + p * p);
 }
@@ -70,7 +85,8 @@ __global__ void k_pointwise_q(float* QH, float* QA, const float* phi, const floa
                               int l, int m, float dt, float l1, float l2, float eps) {
     const int64_t plane = (int64_t)w * h, n = plane * l;
     VOXELS(i, n) {
-        const int x = (int)(i % w), y = (int)((i / w) % h), z = (int)(i / plane);
+        int x, y, z;
+        coords(i, w, h, n, x, y, z);
         const bool valid = x >= m && x < w - m && y >= m && y < h - m && z >= m && z < l - m;
         float qh = 0.f, qa = 0.f;
         if (valid) {
@@ -103,6 +119,62 @@ __global__ void k_conv_same_zero(float* out, const float* in, int w, int h, int 
     }
 }
 
+// The same correlation for two channels at once, with the axis and the tap count
+// fixed at compile time: a 64 x 4 thread block covers 64 x-voxels of 4 rows
+// (y + z*h), rows grid-strided, no 64-bit division; interior voxels take the
+// unrolled taps without bounds checks.  Same operations in the same order as
+// k_conv_same_zero.
+template <int AXIS, int P>
+__global__ __launch_bounds__(256) void k_conv_same_zero2(float* __restrict__ out0, float* __restrict__ out1,
+                                                         const float* __restrict__ in0,
+                                                         const float* __restrict__ in1, int w, int h, int l, Taps t) {
+    constexpr int m = (P - 1) / 2;
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    if (x >= w) return;
+    const int rows = h * l;
+    const int dim = AXIS == 0 ? w : (AXIS == 1 ? h : l);
+    const int64_t stride = AXIS == 0 ? 1 : (AXIS == 1 ? (int64_t)w : (int64_t)w * h);
+    for (int r = blockIdx.y * 4 + threadIdx.y; r < rows; r += gridDim.y * 4) {
+        const int y = r % h, z = r / h;
+        const int c = AXIS == 0 ? x : (AXIS == 1 ? y : z);
+        const int64_t i = (int64_t)r * w + x;
+        float a0 = 0.f, a1 = 0.f;
+        if (c >= m && c + m < dim) {
+            const float* p0 = in0 + i - m * stride;
+            const float* p1 = in1 + i - m * stride;
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                a0 = fmaf(t.g[k], p0[k * stride], a0);
+                a1 = fmaf(t.g[k], p1[k * stride], a1);
+            }
+        } else {
+            for (int k = 0; k < P; ++k) {
+                const int cc = c - m + k;
+                if (cc >= 0 && cc < dim) {
+                    a0 = fmaf(t.g[k], in0[i + (int64_t)(cc - c) * stride], a0);
+                    a1 = fmaf(t.g[k], in1[i + (int64_t)(cc - c) * stride], a1);
+                }
+            }
+        }
+        out0[i] = a0;
+        out1[i] = a1;
+    }
+}
+
+template <int P>
+int conv2_axes(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w, int h, int l,
+               const Taps& taps, hipStream_t s) {
+    const dim3 blk(64, 4);
+    const dim3 grid((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)h * l + 3) / 4, 65535));
+    hipLaunchKernelGGL((k_conv_same_zero2<0, P>), grid, blk, 0, s, Bh, Ba, QH, QA, w, h, l, taps);
+    hipLaunchKernelGGL((k_conv_same_zero2<1, P>), grid, blk, 0, s, t1, t2, (const float*)Bh, (const float*)Ba, w, h,
+                       l, taps);
+    hipLaunchKernelGGL((k_conv_same_zero2<2, P>), grid, blk, 0, s, Bh, Ba, (const float*)t1, (const float*)t2, w, h,
+                       l, taps);
+    RSF_HIP(hipGetLastError());
+    return RSF_OK;
+}
+
 // G_a at every voxel: normalisation Jacobian applied to D^T(kbar) plus D^T(Lbar).
 __global__ void k_stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* gout, int w, int h,
                             int l, float dt, float nu, float mu, float eps) {
@@ -110,7 +182,8 @@ __global__ void k_stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, c
     auto kbar = [&](int64_t q) { return (nu * dt * dirac(phi[q], eps) - mu * dt) * gout[q]; };
     auto lbar = [&](int64_t q) { return mu * dt * gout[q]; };
     VOXELS(j, n) {
-        const int x = (int)(j % w), y = (int)((j / w) % h), z = (int)(j / plane);
+        int x, y, z;
+        coords(j, w, h, n, x, y, z);
         const float nbx = dT_x(kbar, j, x, w, n), nby = dT_yz(kbar, j, y, h, w), nbz = dT_yz(kbar, j, z, l, plane);
         const float lbx = dT_x(lbar, j, x, w, n), lby = dT_yz(lbar, j, y, h, w), lbz = dT_yz(lbar, j, z, l, plane);
         const Grad g = grad_at(phi, x, y, z, w, h, l);
@@ -135,7 +208,8 @@ __global__ __launch_bounds__(kT) void k_finish(float* gin, float* partials, cons
     auto gy = [&](int64_t q) { return Gy[q]; };
     auto gz = [&](int64_t q) { return Gz[q]; };
     VOXELS(i, n) {
-        const int x = (int)(i % w), y = (int)((i / w) % h), z = (int)(i / plane);
+        int x, y, z;
+        coords(i, w, h, n, x, y, z);
         const float st = dT_x(gx, i, x, w, n) + dT_yz(gy, i, y, h, w) + dT_yz(gz, i, z, l, plane);
         // forward curvature and Laplacian at i (same stencil as sweep 2)
         int qlx, qly, qlz;
@@ -226,6 +300,18 @@ int blur_transpose(float* out, const float* in, float* t1, float* t2, int w, int
                        d.m);
     RSF_HIP(hipGetLastError());
     return RSF_OK;
+}
+
+int blur_transpose2(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w, int h,
+                    int l, const Derived& d, hipStream_t s) {
+    switch (d.P) {
+        case 7: return conv2_axes<7>(Bh, Ba, QH, QA, t1, t2, w, h, l, d.taps, s);
+        case 21: return conv2_axes<21>(Bh, Ba, QH, QA, t1, t2, w, h, l, d.taps, s);
+        case 35: return conv2_axes<35>(Bh, Ba, QH, QA, t1, t2, w, h, l, d.taps, s);
+        default:
+            RSF_TRY(blur_transpose(Bh, QH, t1, t2, w, h, l, d, s));
+            return blur_transpose(Ba, QA, t1, t2, w, h, l, d, s);
+    }
 }
 
 int stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* gout, int w, int h, int l,

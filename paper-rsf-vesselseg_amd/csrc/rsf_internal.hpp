@@ -158,6 +158,9 @@ int pointwise_q(float* QH, float* QA, const float* phi, const float* gout, const
                 int m, const rsf_params* p, hipStream_t s);
 int blur_transpose(float* out, const float* in, float* t1, float* t2, int w, int h, int l,
                    const Derived& d, hipStream_t s);
+// Both co-states at once (Bh <- blur^T QH, Ba <- blur^T QA); t1, t2 are scratch.
+int blur_transpose2(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w,
+                    int h, int l, const Derived& d, hipStream_t s);
 int stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* gout, int w, int h,
               int l, const rsf_params* p, hipStream_t s);
 int finish_rows(int64_t n);
