@@ -364,7 +364,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     constexpr int WIN = (RX + P - 1 + 3) & ~3;
     static_assert(NXI % NT == 0 && NYI % NT == 0 && TY % R == 0, "work items must be multiples of the threads");
     __shared__ __align__(16) float sIn[CIN][RH][RWP];
-    __shared__ __align__(16) float sX[CH][RH][TXP];
+    // x-pass output: channel planes CS floats apart, CS = 32 (mod 64), so the two
+    // halves of a wave in the y pass (same rows, neighbouring channels) read
+    // disjoint LDS banks
+    constexpr int CS0 = RH * TXP, CS = CS0 + (((32 - CS0 % 64) % 64) + 64) % 64;
+    __shared__ __align__(16) float sX[CH * CS];
     const int tid = threadIdx.x;
     const uint8_t* img8 = (const uint8_t*)img;
     const float* imgf = (const float*)img;
@@ -484,8 +488,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                         oa[o] = a;
                         ob[o] = b;
                     }
-                    *reinterpret_cast<float4*>(&sX[2 * pr][r][xb]) = make_float4(oa[0], oa[1], oa[2], oa[3]);
-                    *reinterpret_cast<float4*>(&sX[2 * pr + 1][r][xb]) = make_float4(ob[0], ob[1], ob[2], ob[3]);
+                    *reinterpret_cast<float4*>(&sX[(2 * pr) * CS + r * TXP + xb]) = make_float4(oa[0], oa[1], oa[2], oa[3]);
+                    *reinterpret_cast<float4*>(&sX[(2 * pr + 1) * CS + r * TXP + xb]) = make_float4(ob[0], ob[1], ob[2], ob[3]);
                 }
             } else
 #pragma unroll 1
@@ -513,19 +517,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                     for (int kk = 1; kk < P; ++kk) a = fmaf(win[o + kk], taps.g[kk], a);
                     o4[o] = a;
                 }
-                *reinterpret_cast<float4*>(&sX[ch][r][xb]) = make_float4(o4[0], o4[1], o4[2], o4[3]);
+                *reinterpret_cast<float4*>(&sX[ch * CS + r * TXP + xb]) = make_float4(o4[0], o4[1], o4[2], o4[3]);
             }
             __syncthreads();
             // ---- y pass: R outputs of one column, rows r0 .. r0+R-1
 #pragma unroll 1
             for (int k = 0; k < NYI / NT; ++k) {
+                // item e: column cx = e % TX, then channel, then row group, so that
+                // with TX = 32 the two halves of a wave take two channels of the same rows
                 const int e = tid + k * NT;
-                const int ch = e / (TX * (TY / R));
-                const int rem = e - ch * (TX * (TY / R));
-                const int cx = rem % TX, r0 = (rem / TX) * R;
+                const int cx = e % TX, t = e / TX;
+                const int ch = t % CH, r0 = (t / CH) * R;
                 float col[R + P - 1];
 #pragma unroll
-                for (int j = 0; j < R + P - 1; ++j) col[j] = sX[ch][r0 + j][cx];
+                for (int j = 0; j < R + P - 1; ++j) col[j] = sX[ch * CS + (r0 + j) * TXP + cx];
                 const int gx = x0 + cx;
                 float* dst = xy + ch * stride + off + (int64_t)z * plane + gx;
 #pragma unroll
