@@ -110,7 +110,10 @@ int rsf_step_device(rsf_context* ctx, const void* image, int image_dtype, const 
                     float* phi_out, int64_t w, int64_t h, int64_t l, const rsf_params* p);
 /* p->iterations steps in place on phi -- the loop of CUDA_RSF/main.cpp:222-314,
  * captured in a hipGraph.  iter_ms (NULL or p->iterations floats) receives
- * the per-iteration device time, the figure main.cpp:309 prints. */
+ * the per-iteration device time, the figure main.cpp:309 prints.  After the
+ * loop phi is checked: NaN or inf voxels return RSF_ERR_NONFINITE (the
+ * reference carries on silently), with phi left as computed.  The same check
+ * ends rsf_run_batched_device. */
 int rsf_run_device(rsf_context* ctx, const void* image, int image_dtype, float* phi, int64_t w,
                    int64_t h, int64_t l, const rsf_params* p, float* iter_ms);
 /* Host-pointer form of rsf_run_device (used by the rsf CLI). */

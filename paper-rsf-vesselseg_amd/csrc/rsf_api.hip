@@ -164,6 +164,7 @@ struct rsf_context {
     // the preparation (float copy, blur) last computed for it
     const void* held_image = nullptr;
     std::string held_key;
+    DevBuf nonfinite;  // count of NaN/inf voxels found after a run
 };
 
 namespace {
@@ -501,6 +502,37 @@ int capture(rsf_context* c, const Prepared& pr, const void* image, int u8, float
     return RSF_OK;
 }
 
+// Voxels of phi (inside the blocks) whose exponent bits are all ones: NaN or +-inf.
+__global__ void k_nonfinite(const float* __restrict__ phi, const BlockDesc* __restrict__ blocks, int nblocks,
+                            unsigned long long* __restrict__ count) {
+    unsigned long long bad = 0;
+    for (int b = blockIdx.y; b < nblocks; b += gridDim.y) {
+        const BlockDesc bd = blocks[b];
+        const int64_t n = (int64_t)bd.w * bd.h * bd.l;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+            bad += (__float_as_uint(phi[bd.off + i]) & 0x7f800000u) == 0x7f800000u;
+    }
+    if (bad) atomicAdd(count, bad);
+}
+
+// After a run: RSF_ERR_NONFINITE when phi holds NaN or inf (phi is left as computed).
+int check_finite(rsf_context* c, const Prepared& pr, const float* phi) {
+    RSF_TRY(c->nonfinite.ensure(sizeof(unsigned long long)));  // not a graph buffer: no epoch change
+    RSF_HIP(hipMemsetAsync(c->nonfinite.p, 0, sizeof(unsigned long long), c->stream));
+    int64_t nmax = 1;
+    for (auto& b : pr.blocks) nmax = std::max(nmax, (int64_t)b.w * b.h * b.l);
+    const dim3 grid((unsigned)std::min<int64_t>((nmax + 255) / 256, 1024),
+                    (unsigned)std::min<size_t>(pr.blocks.size(), 65535));
+    hipLaunchKernelGGL(k_nonfinite, grid, dim3(256), 0, c->stream, phi, pr.runner->plan.d_blocks.as<BlockDesc>(),
+                       (int)pr.blocks.size(), c->nonfinite.as<unsigned long long>());
+    RSF_HIP(hipGetLastError());
+    unsigned long long bad = 0;
+    RSF_HIP(hipMemcpyAsync(&bad, c->nonfinite.p, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+    RSF_HIP(hipStreamSynchronize(c->stream));
+    if (bad) return set_error(RSF_ERR_NONFINITE, "%llu voxels of phi are NaN or infinite after the run", bad);
+    return RSF_OK;
+}
+
 int run_prepared(rsf_context* c, const Prepared& pr, const void* image, int u8, float* phi, const rsf_params* p,
                  float* iter_ms) {
     RSF_TRY(ensure_workspace(c, pr));
@@ -554,7 +586,7 @@ int run_prepared(rsf_context* c, const Prepared& pr, const void* image, int u8, 
         RSF_HIP(hipStreamSynchronize(c->stream));
         for (int t = 0; t < T; ++t) RSF_HIP(hipEventElapsedTime(&iter_ms[t], c->iter_events[t], c->iter_events[t + 1]));
         RSF_HIP(hipEventElapsedTime(&c->last_ms, c->iter_events[0], c->iter_events[T]));
-        return RSF_OK;
+        return check_finite(c, pr, phi);
     }
     RSF_HIP(hipEventRecord(c->ev0, c->stream));
     for (int k = 0; k < r->graphs.n_launch; ++k) RSF_HIP(hipGraphLaunch(r->graphs.main, c->stream));
@@ -562,7 +594,7 @@ int run_prepared(rsf_context* c, const Prepared& pr, const void* image, int u8, 
     RSF_HIP(hipEventRecord(c->ev1, c->stream));
     RSF_HIP(hipEventSynchronize(c->ev1));
     RSF_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
-    return RSF_OK;
+    return check_finite(c, pr, phi);
 }
 
 int check_ctx(rsf_context* c) {

@@ -268,3 +268,30 @@ def test_config1_full_run_mask_gate(ctx, mode):
         # mask after 100 steps.  The fast path's regrouped sums stay within that
         # FP32-noise band.
         assert dis <= 4e-4, dis
+
+
+def test_nonfinite_phi_is_reported(ctx):
+    """A NaN in phi0 spreads through the blurs; the run reports RSF_ERR_NONFINITE
+    instead of returning garbage as a success (single volume and batched)."""
+    import torch
+
+    img, phi0, _ = scene((30, 32, 36), tubes=4, seed=81)
+    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=3)
+    bad = phi0.copy()
+    bad[15, 16, 18] = np.nan
+    with pytest.raises(rsf_amd.RSFError) as e:
+        ctx.run(to_dev(img), to_dev(bad), p)
+    assert e.value.status == rsf_amd.ERR_NONFINITE and "NaN or infinite" in str(e.value)
+    ctx.run(to_dev(img), to_dev(phi0), p)   # a clean run still succeeds afterwards
+    shapes = [(30, 32, 36), (24, 26, 28)]
+    blocks, total = rsf_amd.pack_blocks(shapes)
+    im = np.zeros(total, np.uint8)
+    ph = np.zeros(total, np.float32)
+    for (off, w, h, l), shp in zip(blocks, shapes):
+        bi, bp, _ = scene(shp, tubes=3, seed=82)
+        im[off:off + bi.size] = bi.ravel()
+        ph[off:off + bp.size] = bp.ravel()
+    ph[blocks[1][0] + 5] = np.inf
+    with pytest.raises(rsf_amd.RSFError) as e:
+        ctx.run_batched(blocks, to_dev(im), to_dev(ph), p)
+    assert e.value.status == rsf_amd.ERR_NONFINITE
