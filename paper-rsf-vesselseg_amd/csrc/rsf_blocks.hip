@@ -136,6 +136,8 @@ int grid_of(int64_t w, int64_t h, int64_t l, int64_t bw, int64_t bh, int64_t bl,
         gd->ax[a].step = B[a] >= N[a] ? N[a] : B[a] - overlap;
         gd->ax[a].nblk = (int)g[a].size();
     }
+    if ((int64_t)g[0].size() * (int64_t)g[1].size() * (int64_t)g[2].size() > (int64_t(1) << 24))
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "more than 2^24 blocks: choose larger blocks");
     gd->overlap = overlap;
     gd->astep = overlap > 1 ? 1.0 / (double)(overlap - 1) : 0.0;
     blocks->clear();

@@ -236,6 +236,7 @@ int rsf_run_npy(rsf_context* c, const char* image_path, const char* seed_path, i
     if (ai.shape.size() != 3) return set_error(RSF_ERR_INVALID_ARGUMENT, "'%s' is not a 3-D array (Z, Y, X)", image_path);
     if (as.shape != ai.shape) return set_error(RSF_ERR_INVALID_ARGUMENT, "initial level set and image shapes differ");
     const int64_t l = ai.shape[0], h = ai.shape[1], w = ai.shape[2], n = w * h * l;
+    if (w <= 0 || h <= 0 || l <= 0) return set_error(RSF_ERR_INVALID_ARGUMENT, "'%s' has an empty dimension", image_path);
     const int dtype = is_u8(ai.descr) ? RSF_DTYPE_U8 : RSF_DTYPE_F32;
     DevBuf img, phi;
     RSF_TRY(img.ensure((size_t)n * (dtype == RSF_DTYPE_U8 ? 1 : 4)));

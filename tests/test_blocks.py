@@ -125,3 +125,8 @@ def test_linspace_restatement_of_the_kernel():
         step = 1.0 / (k - 1) if k > 1 else 0.0
         mine = np.array([1.0 if (k > 1 and i == k - 1) else i * step for i in range(k)])
         assert np.array_equal(mine, np.linspace(0, 1, k)), k
+
+
+def test_grid_rejects_absurd_block_counts():
+    with pytest.raises(rsf_amd.RSFError, match="2\\^24"):
+        rsf_amd.BlockGrid((1024, 1024, 1024), (2, 2, 2), 0)
