@@ -1442,6 +1442,31 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1v<7, true, 32, 16, 512>("p7_lit_32x16"),
         s1v<35, false, 32, 16, 512>("p35_fast_32x16"),
         s1v<35, true, 32, 16, 512>("p35_lit_32x16"),
+        // every other odd tap count up to 35 (sigma up to 5.14): the fused kernel, untuned
+        s1v<9, false, 32, 16, 512>("p9_fast_32x16"),
+        s1v<9, true, 32, 16, 512>("p9_lit_32x16"),
+        s1v<11, false, 32, 16, 512>("p11_fast_32x16"),
+        s1v<11, true, 32, 16, 512>("p11_lit_32x16"),
+        s1v<13, false, 32, 16, 512>("p13_fast_32x16"),
+        s1v<13, true, 32, 16, 512>("p13_lit_32x16"),
+        s1v<15, false, 32, 16, 512>("p15_fast_32x16"),
+        s1v<15, true, 32, 16, 512>("p15_lit_32x16"),
+        s1v<17, false, 32, 16, 512>("p17_fast_32x16"),
+        s1v<17, true, 32, 16, 512>("p17_lit_32x16"),
+        s1v<19, false, 32, 16, 512>("p19_fast_32x16"),
+        s1v<19, true, 32, 16, 512>("p19_lit_32x16"),
+        s1v<23, false, 32, 16, 512>("p23_fast_32x16"),
+        s1v<23, true, 32, 16, 512>("p23_lit_32x16"),
+        s1v<25, false, 32, 16, 512>("p25_fast_32x16"),
+        s1v<25, true, 32, 16, 512>("p25_lit_32x16"),
+        s1v<27, false, 32, 16, 512>("p27_fast_32x16"),
+        s1v<27, true, 32, 16, 512>("p27_lit_32x16"),
+        s1v<29, false, 32, 16, 512>("p29_fast_32x16"),
+        s1v<29, true, 32, 16, 512>("p29_lit_32x16"),
+        s1v<31, false, 32, 16, 512>("p31_fast_32x16"),
+        s1v<31, true, 32, 16, 512>("p31_lit_32x16"),
+        s1v<33, false, 32, 16, 512>("p33_fast_32x16"),
+        s1v<33, true, 32, 16, 512>("p33_lit_32x16"),
     };
     return v;
 }

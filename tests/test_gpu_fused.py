@@ -31,7 +31,11 @@ CASES = [
     (1.0, 0.2, rsf_amd.MODE_LITERAL, np.uint8, (30, 33, 37)),
     (5.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (40, 41, 43)),
     (5.0, 1.0, rsf_amd.MODE_LITERAL, np.uint8, (40, 41, 43)),
-    (1.3, 1.0, rsf_amd.MODE_FAST, np.uint8, (24, 26, 28)),   # P = 9: unfused fallback
+    (1.3, 1.0, rsf_amd.MODE_FAST, np.uint8, (24, 26, 28)),   # P = 9
+    (2.5, 1.0, rsf_amd.MODE_LITERAL, np.uint8, (36, 30, 52)),  # P = 17, u = 12 (effective sigma sqrt 6)
+    (2.5, 0.2, rsf_amd.MODE_FAST, np.float32, (33, 35, 38)),
+    (4.8, 1.0, rsf_amd.MODE_LITERAL, np.uint8, (40, 41, 43)),  # P = 33
+    (5.3, 1.0, rsf_amd.MODE_FAST, np.uint8, (40, 41, 43)),     # P = 37: unfused fallback
     # w a multiple of 4: the vector interior of sweep 2
     (3.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (40, 44, 48)),
     (3.0, 0.2, rsf_amd.MODE_LITERAL, np.uint8, (36, 30, 52)),
@@ -56,7 +60,7 @@ def test_fused_one_step_matches_oracle(ctx, sigma, eps, mode, dtype, shape):
     ref = oracle.step(phi0, img.astype(np.float32), oracle_params(p))
     err, tol = one_step_gate(got, ref)
     assert err <= tol, (err, tol)
-    if mode == rsf_amd.MODE_LITERAL and sigma != 1.3:
+    if mode == rsf_amd.MODE_LITERAL:
         # the literal path performs the oracle's operations in the oracle's order
         assert np.array_equal(got, ref), err
     else:
@@ -295,3 +299,33 @@ def test_nonfinite_phi_is_reported(ctx):
     with pytest.raises(rsf_amd.RSFError) as e:
         ctx.run_batched(blocks, to_dev(im), to_dev(ph), p)
     assert e.value.status == rsf_amd.ERR_NONFINITE
+
+
+@pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
+def test_batched_at_other_tap_counts(ctx, mode):
+    """sigma 2.5 (P = 17): the batched launch equals the blocks run one by one, and in
+    literal mode each block equals the oracle's run on that block."""
+    import torch
+
+    shapes = [(30, 33, 36), (24, 40, 28), (36, 26, 32)]
+    blocks, total = rsf_amd.pack_blocks(shapes)
+    img = np.empty(total, np.uint8)
+    phi = np.empty(total, np.float32)
+    for k, ((off, w, h, l), shp) in enumerate(zip(blocks, shapes)):
+        bi, bp, _ = scene(shp, tubes=3, seed=90 + k)
+        img[off:off + bi.size] = bi.ravel()
+        phi[off:off + bp.size] = bp.ravel()
+    p = rsf_amd.make_params(sigma=2.5, epsilon=1.0, iterations=4, mode=mode)
+    d_img = to_dev(img)
+    a, b = to_dev(phi), to_dev(phi)
+    ctx.run_batched(blocks, d_img, a, p)
+    ctx.run_sequential(blocks, d_img, b, p)
+    torch.cuda.synchronize()
+    A = a.cpu().numpy()
+    assert np.array_equal(A, b.cpu().numpy())
+    if mode == rsf_amd.MODE_LITERAL:
+        for (off, w, h, l), shp in zip(blocks, shapes):
+            n = w * h * l
+            ref = oracle.run(phi[off:off + n].reshape(shp), img[off:off + n].reshape(shp).astype(np.float32),
+                             oracle_params(p), 4)
+            assert np.array_equal(A[off:off + n].reshape(shp), ref)
