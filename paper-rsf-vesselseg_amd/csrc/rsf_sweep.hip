@@ -341,7 +341,7 @@ __device__ __forceinline__ void sweep1_channels(float i, float t, float (&c)[LIT
 // outputs to LDS (RX = 4 per work item), the y pass R outputs per work item
 // from R+P-1 LDS reads.  The work items of both passes are exact multiples of
 // the thread count.  Channel ch of the output goes to xy + ch * stride.
-template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, bool HI = false>
+template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, bool HI = false, int RX = 4>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_kernel(
     const float* __restrict__ tr, const void* __restrict__ img, int img_u8, float* __restrict__ xy, int64_t stride,
     const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc, Taps taps) {
@@ -358,7 +358,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     constexpr int QW = RWP / 4;            // float4 groups per LDS input row
     constexpr int NQ = RH * QW;
     constexpr int PF = (NQ + NT - 1) / NT;
-    constexpr int RX = 4, XG = TX / RX;
+    constexpr int XG = TX / RX;  // RX: x-pass outputs per work item (4 or 8)
+    static_assert(RX == 4 || RX == 8, "x-pass outputs per item");
     constexpr int NXI = (HI ? 2 : CH) * RH * XG;  // x-pass work items
     constexpr int NYI = CH * TX * (TY / R);  // y-pass work items
     constexpr int WIN = (RX + P - 1 + 3) & ~3;
@@ -488,8 +489,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                         oa[o] = a;
                         ob[o] = b;
                     }
-                    *reinterpret_cast<float4*>(&sX[(2 * pr) * CS + r * TXP + xb]) = make_float4(oa[0], oa[1], oa[2], oa[3]);
-                    *reinterpret_cast<float4*>(&sX[(2 * pr + 1) * CS + r * TXP + xb]) = make_float4(ob[0], ob[1], ob[2], ob[3]);
+#pragma unroll
+                    for (int o = 0; o < RX; o += 4) {
+                        *reinterpret_cast<float4*>(&sX[(2 * pr) * CS + r * TXP + xb + o]) =
+                            make_float4(oa[o], oa[o + 1], oa[o + 2], oa[o + 3]);
+                        *reinterpret_cast<float4*>(&sX[(2 * pr + 1) * CS + r * TXP + xb + o]) =
+                            make_float4(ob[o], ob[o + 1], ob[o + 2], ob[o + 3]);
+                    }
                 }
             } else
 #pragma unroll 1
@@ -517,7 +523,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                     for (int kk = 1; kk < P; ++kk) a = fmaf(win[o + kk], taps.g[kk], a);
                     o4[o] = a;
                 }
-                *reinterpret_cast<float4*>(&sX[ch * CS + r * TXP + xb]) = make_float4(o4[0], o4[1], o4[2], o4[3]);
+#pragma unroll
+                for (int o = 0; o < RX; o += 4)
+                    *reinterpret_cast<float4*>(&sX[ch * CS + r * TXP + xb + o]) =
+                        make_float4(o4[o], o4[o + 1], o4[o + 2], o4[o + 3]);
             }
             __syncthreads();
             // ---- y pass: R outputs of one column, rows r0 .. r0+R-1
@@ -1417,10 +1426,11 @@ Sweep1Variant s1split(const void* xy_fn, const char* name) {
     return v;
 }
 
-template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D, bool HI = false>
+template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D, bool HI = false,
+          int RX = 4>
 Sweep1Variant s1s(const char* name) {
-    return s1split<P, LIT, TX, TY, NT, TXC, TYC, D>((const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI>,
-                                                   name);
+    return s1split<P, LIT, TX, TY, NT, TXC, TYC, D>(
+        (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX>, name);
 }
 }  // namespace
 
@@ -1431,6 +1441,7 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
         s1s<21, true, 32, 60, 256, 10, 2, 256, 1, 3, true>("p21_lit_hi_32x60r10"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
+        s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true, 8>("p21_lit_hi_32x44x8"),
         s1s<21, true, 32, 28, 256, 7, 4, 256, 1, 3>("p21_lit_split_32x28w4"),
         s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, true>("p21_lit_hi_32x60r15"),
         s1s<21, true, 64, 28, 512, 7, 4, 256, 1, 3>("p21_lit_split_64x28w4"),
