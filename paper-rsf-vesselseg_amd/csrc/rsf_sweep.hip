@@ -341,7 +341,8 @@ __device__ __forceinline__ void sweep1_channels(float i, float t, float (&c)[LIT
 // outputs to LDS (RX = 4 per work item), the y pass R outputs per work item
 // from R+P-1 LDS reads.  The work items of both passes are exact multiples of
 // the thread count.  Channel ch of the output goes to xy + ch * stride.
-template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, bool HI = false, int RX = 4>
+template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, bool HI = false, int RX = 4, int UX = 1,
+          int UY = 1>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_kernel(
     const float* __restrict__ tr, const void* __restrict__ img, int img_u8, float* __restrict__ xy, int64_t stride,
     const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc, Taps taps) {
@@ -360,6 +361,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     constexpr int PF = (NQ + NT - 1) / NT;
     constexpr int XG = TX / RX;  // RX: x-pass outputs per work item (4 or 8)
     static_assert(RX == 4 || RX == 8, "x-pass outputs per item");
+    // UX, UY: unroll of the x- and y-pass item loops.  The LDS footprint caps the
+    // kernel at two workgroups per CU, so registers are free to hold the next
+    // item's loads in flight while the current item computes.
     constexpr int NXI = (HI ? 2 : CH) * RH * XG;  // x-pass work items
     constexpr int NYI = CH * TX * (TY / R);  // y-pass work items
     constexpr int WIN = (RX + P - 1 + 3) & ~3;
@@ -455,7 +459,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             __syncthreads();
             // ---- x pass (one item at a time: registers, not latency, are the limit)
             if constexpr (HI) {
-#pragma unroll 1
+#pragma unroll UX
                 for (int k = 0; k < NXI / NT; ++k) {
                     const int e = tid + k * NT;
                     const int pr = e / (RH * XG);
@@ -498,7 +502,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                     }
                 }
             } else
-#pragma unroll 1
+#pragma unroll UX
             for (int k = 0; k < NXI / NT; ++k) {
                 const int e = tid + k * NT;
                 const int ch = e / (RH * XG);
@@ -530,7 +534,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             }
             __syncthreads();
             // ---- y pass: R outputs of one column, rows r0 .. r0+R-1
-#pragma unroll 1
+#pragma unroll UY
             for (int k = 0; k < NYI / NT; ++k) {
                 // item e: column cx = e % TX, then channel, then row group, so that
                 // with TX = 32 the two halves of a wave take two channels of the same rows
@@ -1427,10 +1431,10 @@ Sweep1Variant s1split(const void* xy_fn, const char* name) {
 }
 
 template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D, bool HI = false,
-          int RX = 4>
+          int RX = 4, int UX = 1, int UY = 1>
 Sweep1Variant s1s(const char* name) {
     return s1split<P, LIT, TX, TY, NT, TXC, TYC, D>(
-        (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX>, name);
+        (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY>, name);
 }
 }  // namespace
 
