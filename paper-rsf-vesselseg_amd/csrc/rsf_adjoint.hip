@@ -121,9 +121,9 @@ __global__ void k_conv_same_zero(float* out, const float* in, int w, int h, int 
 
 // The same correlation for two channels at once, with the axis and the tap count
 // fixed at compile time: a 64 x 4 thread block covers 64 x-voxels of 4 rows
-// (y + z*h), rows grid-strided, no 64-bit division; interior voxels take the
-// unrolled taps without bounds checks.  Same operations in the same order as
-// k_conv_same_zero.
+// (y + z*h), rows grid-strided, no 64-bit division; along y and z interior rows
+// take the unrolled taps without bounds checks, along x every lane runs the
+// same masked taps.  The taps are summed in the order of k_conv_same_zero.
 template <int AXIS, int P>
 __global__ __launch_bounds__(256) void k_conv_same_zero2(float* __restrict__ out0, float* __restrict__ out1,
                                                          const float* __restrict__ in0,
@@ -139,7 +139,18 @@ __global__ __launch_bounds__(256) void k_conv_same_zero2(float* __restrict__ out
         const int c = AXIS == 0 ? x : (AXIS == 1 ? y : z);
         const int64_t i = (int64_t)r * w + x;
         float a0 = 0.f, a1 = 0.f;
-        if (c >= m && c + m < dim) {
+        if constexpr (AXIS == 0) {
+            // along x the lanes of a wave straddle the block's ends: no branch, the
+            // out-of-range taps read a clamped neighbour and add g * 0
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const int cc = c - m + k;
+                const bool in_range = cc >= 0 && cc < dim;
+                const int64_t q = i + (int64_t)(min(max(cc, 0), dim - 1) - c);
+                a0 = fmaf(t.g[k], in_range ? in0[q] : 0.f, a0);
+                a1 = fmaf(t.g[k], in_range ? in1[q] : 0.f, a1);
+            }
+        } else if (c >= m && c + m < dim) {
             const float* p0 = in0 + i - m * stride;
             const float* p1 = in1 + i - m * stride;
 #pragma unroll
@@ -239,18 +250,28 @@ __global__ __launch_bounds__(kT) void k_finish(float* gin, float* partials, cons
         s2 += dt * d * kap * u;
         s3 += dt * (lap - kap) * u;
     }
-    __shared__ float red[4][kT];
-    red[0][threadIdx.x] = s0;
-    red[1][threadIdx.x] = s1;
-    red[2][threadIdx.x] = s2;
-    red[3][threadIdx.x] = s3;
-    __syncthreads();
-    for (int o = kT / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
-        __syncthreads();
+    // fixed-order reduction: a shuffle tree within each 64-lane wave, then the waves
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_down(s0, o, 64);
+        s1 += __shfl_down(s1, o, 64);
+        s2 += __shfl_down(s2, o, 64);
+        s3 += __shfl_down(s3, o, 64);
     }
-    if (threadIdx.x < 4) partials[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    constexpr int kW = kT / 64;
+    __shared__ float red[4][kW];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wv] = s0;
+        red[1][wv] = s1;
+        red[2][wv] = s2;
+        red[3][wv] = s3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        float v = red[threadIdx.x][0];
+        for (int q = 1; q < kW; ++q) v += red[threadIdx.x][q];
+        partials[4 * blockIdx.x + threadIdx.x] = v;
+    }
 }
 
 // grad_params[q] += sum over the partial rows, in double.  One workgroup: thread
@@ -323,13 +344,17 @@ int stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* go
     return RSF_OK;
 }
 
-int finish_rows(int64_t n) { return (int)grid_for(n).x; }
+// the finishing kernel runs at most 1024 workgroups per block, each thread
+// accumulating several voxels before the reduction
+dim3 grid_finish(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kT - 1) / kT, 1024))); }
+
+int finish_rows(int64_t n) { return (int)grid_finish(n).x; }
 
 int finish(float* gin, float* partials, const float* phi, const float* gout, const float* Gx, const float* Gy,
            const float* Gz, const float* Bh, const float* Ba, const void* img, int u8, const float* fo,
            const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s) {
     const int64_t n = (int64_t)w * h * l;
-    hipLaunchKernelGGL(k_finish, grid_for(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, Bh, Ba, img, u8,
+    hipLaunchKernelGGL(k_finish, grid_finish(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, Bh, Ba, img, u8,
                        fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2);
     RSF_HIP(hipGetLastError());
     return RSF_OK;
