@@ -650,7 +650,7 @@ void rsf_params_default(rsf_params* p) {
     p->nu = 0.0001f;      // :96
     p->sigma2 = 0.0f;     // :97
     p->epsilon = 0.2f;    // kernel.cu:258
-    p->mode = RSF_MODE_FAST;
+    p->mode = RSF_MODE_LITERAL;  // the reference's arithmetic; RSF_MODE_FAST is opt-in
 }
 
 int rsf_taps(float sigma, float* taps, int capacity, int* ntaps, int* pad) {

@@ -55,7 +55,7 @@ const Opt kOpts[] = {
     {"lambda2", true, "weight of the inside fitting energy (default: --e)", nullptr},
     {"epsilon", true, "Heaviside/Dirac width", "0.2"},
     {"phi0", true, "initial level set (.npy, used instead of sdf(binary - 13))", nullptr},
-    {"mode", true, "fast | literal (four blurred channels in reference order)", "fast"},
+    {"mode", true, "literal (the reference's arithmetic, bit for bit) | fast (two blurred channels)", "literal"},
     {"device", true, "HIP device index", "0"},
     {"block", true, "run as overlapping blocks of this size: N or W,H,L", nullptr},
     {"overlap", true, "overlap of neighbouring blocks with --block", "32"},

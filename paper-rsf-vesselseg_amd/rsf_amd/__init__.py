@@ -60,7 +60,7 @@ class Block(C.Structure):
 
 
 def make_params(sigma=3.0, lambda1=1.0, lambda2=None, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0,
-                sigma2=0.0, iterations=1, mode=MODE_FAST) -> Params:
+                sigma2=0.0, iterations=1, mode=MODE_LITERAL) -> Params:
     if lambda2 is None:
         lambda2 = lambda1
     return Params(sigma, lambda1, lambda2, nu, mu, dt, epsilon, sigma2, int(iterations), int(mode))

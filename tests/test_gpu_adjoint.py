@@ -20,7 +20,9 @@ def ctx():
 
 
 def _params(sigma, **kw):
-    base = dict(sigma=sigma, lambda1=1.0, lambda2=1.0, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0)
+    # FAST unless a case asks for LITERAL: the mode the unrolled training path runs
+    base = dict(sigma=sigma, lambda1=1.0, lambda2=1.0, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0,
+                mode=rsf_amd.MODE_FAST)
     base.update(kw)
     return rsf_amd.make_params(**base)
 

@@ -99,11 +99,12 @@ def test_run_matches_oracle_mask(ctx, mode):
     assert float(np.median(diff)) <= 1e-5 * float(np.abs(ref).max())
 
 
-def test_graph_run_equals_eager_steps(ctx):
+@pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
+def test_graph_run_equals_eager_steps(ctx, mode):
     import torch
 
     img, phi0, _ = scene((40, 42, 44), tubes=5, seed=4)
-    p1 = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=1)
+    p1 = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=1, mode=mode)
     d_img = to_dev(img)
     a, b = to_dev(phi0), torch.empty_like(to_dev(phi0))
     eager = {}
@@ -114,7 +115,7 @@ def test_graph_run_equals_eager_steps(ctx):
             eager[t] = a.cpu().numpy()
     # T = 5: one 4-iteration graph + remainder; T = 70: 64-iteration graph + 6
     for T in (5, 70):
-        pT = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=T)
+        pT = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=T, mode=mode)
         for _ in range(2):  # the second call replays the cached graph
             d_phi = to_dev(phi0)
             ctx.run(d_img, d_phi, pT)
@@ -131,7 +132,8 @@ def test_per_iteration_timing_host_path(ctx):
         assert mask_disagreement(out, ref) <= 1e-4
 
 
-def test_batched_equals_sequential_blocks(ctx):
+@pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
+def test_batched_equals_sequential_blocks(ctx, mode):
     import torch
 
     rng = np.random.default_rng(6)
@@ -143,7 +145,7 @@ def test_batched_equals_sequential_blocks(ctx):
         bi, bp, _ = scene(shp, tubes=3, seed=100 + k)
         img[off:off + bi.size] = bi.ravel()
         phi[off:off + bp.size] = bp.ravel()
-    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=6)
+    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=6, mode=mode)
     d_img = to_dev(img)
     a, b = to_dev(phi), to_dev(phi)
     ctx.run_batched(blocks, d_img, a, p)

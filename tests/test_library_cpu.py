@@ -43,6 +43,8 @@ def test_reference_defaults():
     assert (p.sigma, p.iterations, p.mu) == (5.0, 30, 1.0)
     assert abs(p.dt - 0.1) < 1e-7 and abs(p.lambda1 - 0.1) < 1e-7 and abs(p.nu - 1e-4) < 1e-10
     assert p.sigma2 == 0.0 and abs(p.epsilon - 0.2) < 1e-7
+    assert p.mode == rsf_amd.MODE_LITERAL   # the reference's arithmetic unless asked otherwise
+    assert rsf_amd.make_params().mode == rsf_amd.MODE_LITERAL
 
 
 @pytest.mark.parametrize("sigma", [1.0, 2.5, 3.0, 5.0])
