@@ -126,6 +126,10 @@ def batched_run(rsf_amd, torch, ctx, steps, mode, nblocks=64):
     d_img = torch.from_numpy(img).to(dev)
     d_phi0 = torch.from_numpy(phi).to(dev)
     p = params(rsf_amd, steps, mode)
+    # the batched launch runs with the tile shapes the autotuner picks for this block set
+    # (untimed); the per-block schedule takes each volume's defaults
+    choice = ctx.autotune(blocks, rsf_amd.DTYPE_U8, params(rsf_amd, 1, mode))
+    log(f"batched autotune: {choice}")
     out = {}
     for name, fn in (("batched", ctx.run_batched), ("per_block", ctx.run_sequential)):
         d_phi = d_phi0.clone()
@@ -147,6 +151,7 @@ def batched_run(rsf_amd, torch, ctx, steps, mode, nblocks=64):
         "value": round(gv(out["batched"]), 3),
         "plain_per_block": round(gv(out["per_block"]), 3),
         "speedup": round(out["per_block"] / out["batched"], 3),
+        "kernels": choice,
         "results_identical": same,
     }
 
