@@ -187,6 +187,14 @@ int rsf_reassemble_device(rsf_context* ctx, const float* packed, int64_t w, int6
 int rsf_run_volume_blocks_device(rsf_context* ctx, const void* image, int image_dtype, float* phi, int64_t w,
                                  int64_t h, int64_t l, int64_t bw, int64_t bh, int64_t bl, int64_t overlap,
                                  const rsf_params* p);
+/* The seed producer's vote (Unet/run_model.py:29-35,86-91): pred holds
+ * nz*ny*nx tile outputs of tz*ty*tx floats, tile t = (i*ny + j)*nx + k starting
+ * at (x0[k], y0[j], z0[i]) (starts ascending per axis, tiles inside the volume).
+ * Every voxel gets the mean of its covering tiles' outputs, summed in float64 in
+ * the order of t, into prob, and prob >= 0.5 into mask (0 where no tile covers). */
+int rsf_tile_vote_device(rsf_context* ctx, const float* pred, int64_t tz, int64_t ty, int64_t tx,
+                         const int64_t* z0, int nz, const int64_t* y0, int ny, const int64_t* x0, int nx,
+                         int64_t w, int64_t h, int64_t l, double* prob, uint8_t* mask);
 /* Host-pointer forms of rsf_run_batched_device and rsf_run_volume_blocks_device
  * (used by the rsf CLI's --batch and --block modes). */
 int rsf_run_batched_host(rsf_context* ctx, const rsf_block* blocks, int nblocks, const void* image,

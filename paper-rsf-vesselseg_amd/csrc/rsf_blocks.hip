@@ -315,3 +315,86 @@ int rsf_run_volume_blocks_host(rsf_context* c, const void* image, int image_dtyp
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ tile votes
+// The vote of the seed producer (Unet/run_model.py:29-35,86-91): every voxel
+// averages the outputs of the tiles covering it, the sum taken in float64 in the
+// tiles' loop order (i along z, then j along y, then k along x), then >= 0.5.
+namespace rsf {
+namespace {
+
+__global__ void k_tile_vote(const float* __restrict__ pred, int tz, int ty, int tx, int ny, int nx,
+                            const int2* __restrict__ cz, const int2* __restrict__ cy, const int2* __restrict__ cx,
+                            const int64_t* __restrict__ z0, const int64_t* __restrict__ y0,
+                            const int64_t* __restrict__ x0, int64_t w, int64_t h, int64_t n,
+                            double* __restrict__ prob, uint8_t* __restrict__ mask) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t x = v % w, y = (v / w) % h, z = v / (w * h);
+        const int2 rz = cz[z], ry = cy[y], rx = cx[x];  // covering tile ranges [lo, hi)
+        double acc = 0.0;
+        int cnt = 0;
+        for (int i = rz.x; i < rz.y; ++i)
+            for (int j = ry.x; j < ry.y; ++j)
+                for (int k = rx.x; k < rx.y; ++k) {
+                    const int64_t t = ((int64_t)i * ny + j) * nx + k;
+                    acc = acc + (double)pred[((t * tz + (z - z0[i])) * ty + (y - y0[j])) * tx + (x - x0[k])];
+                    ++cnt;
+                }
+        const double p = cnt > 0 ? acc / (double)cnt : 0.0;
+        prob[v] = p;
+        mask[v] = p >= 0.5 ? 1 : 0;
+    }
+}
+
+// [lo, hi) of the tiles (non-decreasing starts, one size) covering each coordinate
+int cover_ranges(const int64_t* s, int nt, int64_t size, int64_t n, std::vector<int2>* out) {
+    out->assign((size_t)n, int2{0, 0});
+    for (int t = 0; t < nt; ++t) {
+        if (s[t] < 0 || s[t] + size > n || (t > 0 && s[t] < s[t - 1]))
+            return set_error(RSF_ERR_INVALID_ARGUMENT, "tile starts must be ascending and inside the volume");
+    }
+    for (int64_t c = 0; c < n; ++c) {
+        int lo = 0;
+        while (lo < nt && s[lo] + size <= c) ++lo;
+        int hi = lo;
+        while (hi < nt && s[hi] <= c) ++hi;
+        (*out)[(size_t)c] = int2{lo, hi};
+    }
+    return RSF_OK;
+}
+
+}  // namespace
+}  // namespace rsf
+
+extern "C" int rsf_tile_vote_device(rsf_context* c, const float* pred, int64_t tz, int64_t ty, int64_t tx,
+                                    const int64_t* z0, int nz, const int64_t* y0, int ny, const int64_t* x0, int nx,
+                                    int64_t w, int64_t h, int64_t l, double* prob, uint8_t* mask) {
+    hipStream_t s;
+    RSF_TRY(context_stream(c, &s));
+    if (!pred || !z0 || !y0 || !x0 || !prob || !mask) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (tz <= 0 || ty <= 0 || tx <= 0 || nz <= 0 || ny <= 0 || nx <= 0 || w <= 0 || h <= 0 || l <= 0)
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "tile and volume dims must be positive");
+    std::vector<int2> rz, ry, rx;
+    RSF_TRY(cover_ranges(z0, nz, tz, l, &rz));
+    RSF_TRY(cover_ranges(y0, ny, ty, h, &ry));
+    RSF_TRY(cover_ranges(x0, nx, tx, w, &rx));
+    DevBuf d_rz, d_ry, d_rx, d_z0, d_y0, d_x0;
+    auto up = [&](DevBuf& d, const void* src, size_t bytes) -> int {
+        RSF_TRY(d.ensure(bytes));
+        RSF_HIP(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, s));
+        return RSF_OK;
+    };
+    RSF_TRY(up(d_rz, rz.data(), rz.size() * sizeof(int2)));
+    RSF_TRY(up(d_ry, ry.data(), ry.size() * sizeof(int2)));
+    RSF_TRY(up(d_rx, rx.data(), rx.size() * sizeof(int2)));
+    RSF_TRY(up(d_z0, z0, (size_t)nz * sizeof(int64_t)));
+    RSF_TRY(up(d_y0, y0, (size_t)ny * sizeof(int64_t)));
+    RSF_TRY(up(d_x0, x0, (size_t)nx * sizeof(int64_t)));
+    const int64_t n = w * h * l;
+    hipLaunchKernelGGL(k_tile_vote, dim3(grid1d(n)), dim3(256), 0, s, pred, (int)tz, (int)ty, (int)tx, ny, nx,
+                       d_rz.as<int2>(), d_ry.as<int2>(), d_rx.as<int2>(), d_z0.as<int64_t>(), d_y0.as<int64_t>(),
+                       d_x0.as<int64_t>(), w, h, n, prob, mask);
+    RSF_HIP(hipGetLastError());
+    RSF_HIP(hipStreamSynchronize(s));
+    return RSF_OK;
+}

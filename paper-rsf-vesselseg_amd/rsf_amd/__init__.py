@@ -113,6 +113,7 @@ _SIGS = {
     "rsf_npy_load_device": (_I, [_P, C.c_char_p, _P, _I, _I64]),
     "rsf_npy_save_device": (_I, [_P, C.c_char_p, _P, _I64, _I64, _I64]),
     "rsf_run_npy": (_I, [_P, C.c_char_p, C.c_char_p, _I, C.c_char_p, _PP, _P]),
+    "rsf_tile_vote_device": (_I, [_P, _P, _I64, _I64, _I64, _P, _I, _P, _I, _P, _I, _I64, _I64, _I64, _P, _P]),
     "rsf_synth_tubes": (_I, [_P, _P, _I64, _I64, _I64, _I, _F, _F, _F, _F, C.c_uint64]),
     "rsf_initial_phi": (_I, [_P, _P, _I64, _I64, _I64, _F, _F]),
     "rsf_sdf_from_seed": (_I, [_P, _P, _I64, _I64, _I64, _F]),
@@ -362,6 +363,16 @@ class Context:
 
     def release_image(self):
         check(lib().rsf_context_release_image(self._h))
+
+    def tile_vote(self, pred, tile, starts, shape, prob, mask):
+        """rsf_tile_vote_device: pred (T, tz, ty, tx) float32, tile = (tz, ty, tx),
+        starts = (z0s, y0s, x0s) per axis, shape = (l, h, w); prob float64 and mask
+        uint8 device tensors of that shape receive the vote."""
+        z0, y0, x0 = (np.ascontiguousarray(a, np.int64) for a in starts)
+        l, h, w = shape
+        _sync(pred, prob, mask)
+        check(lib().rsf_tile_vote_device(self._h, _ptr(pred), *tile, z0.ctypes.data, len(z0), y0.ctypes.data,
+                                         len(y0), x0.ctypes.data, len(x0), w, h, l, _ptr(prob), _ptr(mask)))
 
     def npy_load(self, path, dst):
         """Stream a 3-D .npy file into the device tensor dst (uint8 as is, else float32)."""

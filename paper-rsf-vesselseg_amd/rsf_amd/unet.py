@@ -159,14 +159,23 @@ def vote(pred: torch.Tensor, shape, slices):
     """count / total of run_model.py:86-91 from the tile outputs `pred` (T, 16, 64, 64).
 
     Every voxel sums its covering tiles' outputs in float64 in the reference's
-    loop order (an exact gather per tile combination, no atomics), so the result
-    is the reference's bit for bit given the same tile outputs.  Returns
-    (probability float64, map bool) on pred's device; voxels no tile covers get 0."""
+    loop order, so the result is the reference's bit for bit given the same tile
+    outputs.  Returns (probability float64, map bool) on pred's device; voxels no
+    tile covers get 0.  Device tensors go through the HIP kernel
+    (rsf_tile_vote_device); host tensors through an exact gather per tile
+    combination in torch."""
     l, h, w = shape
     dev = pred.device
     nz = math.ceil(l / NUM_PICS)
     ny = math.ceil(h / (HEIGHT // STEP_RATIO))
     nx = math.ceil(w / (WIDTH // STEP_RATIO))
+    if dev.type == "cuda":
+        ctx = _vote_context(dev)
+        prob = torch.empty(shape, dtype=torch.float64, device=dev)
+        mask = torch.empty(shape, dtype=torch.uint8, device=dev)
+        starts = ([s[0] for s in slices[::ny * nx]], [s[2] for s in slices[:ny * nx:nx]], [s[4] for s in slices[:nx]])
+        ctx.tile_vote(pred.contiguous(), (NUM_PICS, HEIGHT, WIDTH), starts, shape, prob, mask)
+        return prob, mask.bool()
     zi, zo = _cover([s[0] for s in slices[::ny * nx]], [s[1] for s in slices[::ny * nx]], l, dev)
     yi, yo = _cover([s[2] for s in slices[:ny * nx:nx]], [s[3] for s in slices[:ny * nx:nx]], h, dev)
     xi, xo = _cover([s[4] for s in slices[:nx]], [s[5] for s in slices[:nx]], w, dev)
@@ -186,6 +195,18 @@ def vote(pred: torch.Tensor, shape, slices):
     prob = torch.where(total > 0, count / torch.where(total > 0, total, torch.ones_like(total)),
                        torch.zeros_like(count))
     return prob, prob >= 0.5
+
+
+_contexts: dict = {}
+
+
+def _vote_context(dev):
+    from . import Context
+
+    idx = dev.index if dev.index is not None else 0
+    if idx not in _contexts:
+        _contexts[idx] = Context(idx)
+    return _contexts[idx]
 
 
 def segment(model, volume: np.ndarray, mode: str | None = None, device="cuda", batch: int = 64):
