@@ -435,6 +435,8 @@ def npy_info(path):
     shape = (C.c_int64 * 8)()
     nd, u8 = C.c_int(), C.c_int()
     check(lib().rsf_npy_info(os.fsencode(path), shape, C.byref(nd), C.byref(u8)))
+    if nd.value > 8:
+        raise ValueError(f"{path}: {nd.value} dimensions (at most 8 supported)")
     return tuple(shape[k] for k in range(nd.value)), bool(u8.value)
 
 
