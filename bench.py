@@ -235,6 +235,27 @@ def io_run(rsf_amd, torch, ctx):
         np.save(os.path.join(d, "out2.npy"), src.cpu().numpy())
         tn = time.perf_counter() - t0
         out["save_phi"] = {"streamed_gbs": round(4 * n / ts / 1e9, 2), "numpy_gbs": round(4 * n / tn / 1e9, 2)}
+        del src
+        # the whole rsf run file to file (rsf_run_npy, what the rsf executable calls): the
+        # 512^3 phantom and its 0/255 seed streamed in, phi0 = sdf(seed - 13) on the device,
+        # 100 literal iterations, phi streamed out
+        shp = (512, 512, 512)
+        img, mask = rsf_amd.synth_tubes(shp, tubes=300, seed=2)
+        np.save(os.path.join(d, "e2e_img.npy"), img)
+        np.save(os.path.join(d, "e2e_seed.npy"), (mask * 255).astype(np.uint8))
+        del img, mask
+        T = 100
+        p = params(rsf_amd, T, rsf_amd.MODE_LITERAL)
+        paths = [os.path.join(d, f) for f in ("e2e_img.npy", "e2e_seed.npy", "e2e_phi.npy")]
+        ctx.run_npy(*paths, p)   # warm: graphs, workspace
+        t0 = time.perf_counter()
+        ms = ctx.run_npy(*paths, p, per_iteration=True)
+        wall = time.perf_counter() - t0
+        nv = 512 ** 3
+        out["file_to_file"] = {"volume": [512, 512, 512], "iterations": T, "wall_s": round(wall, 3),
+                               "loop_s": round(float(ms.sum()) / 1e3, 3),
+                               "value": round(nv * T / wall / 1e9, 3), "unit": "Gvoxel-iterations/s",
+                               "note": "includes .npy reads, PCIe, phi0 on the device and the .npy write"}
     finally:
         shutil.rmtree(d, ignore_errors=True)
     log(f"io: {out}")
