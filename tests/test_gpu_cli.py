@@ -82,3 +82,34 @@ def test_cli_batch_manifest_equals_oracle_per_block(tmp_path):
     assert r.stdout.count("It takes") == 3
     for k, ref in enumerate(refs):
         assert np.array_equal(np.load(tmp_path / "blocks" / f"phi{k}.npy"), ref), k
+
+
+def test_block_files_round_trip_equals_oracle(tmp_path):
+    """disassemble_npy -> rsf --batch -> reassemble_npy (the reference's
+    disassemble / rsfbatch / reassemble workflow, README.md:56-59) equals the oracle
+    evolving every block from its own seed, blended by the oracle's blend."""
+    from rsf_amd import blockfiles
+
+    shape, bs, k = (40, 44, 50), (30, 26, 24), 6
+    img, seed = _seed(shape, 33)
+    np.save(tmp_path / "vol.npy", img)
+    np.save(tmp_path / "seed.npy", seed)
+    d = tmp_path / "blocks"
+    ctx = rsf_amd.Context(0)
+    desc = blockfiles.disassemble_npy(str(tmp_path / "vol.npy"), str(d), bs, k, str(tmp_path / "seed.npy"),
+                                      min_size=7, context=ctx)
+    r = _cli("--batch", str(d / "manifest.txt"), *ARGS)
+    assert r.returncode == 0, r.stderr
+    blockfiles.reassemble_npy(str(d), str(tmp_path / "phi.npy"), context=ctx)
+    ctx.close()
+    got = np.load(tmp_path / "phi.npy")
+    blocks, origins, axes = ob.block_grid(shape, bs, k, 7)
+    assert len(desc["blocks"]) == len(blocks)
+    packed = np.empty(sum(b[1] * b[2] * b[3] for b in blocks), np.float32)
+    for (off, w, h, l), (x0, y0, z0) in zip(blocks, origins):
+        ib = img[z0:z0 + l, y0:y0 + h, x0:x0 + w]
+        sb = seed[z0:z0 + l, y0:y0 + h, x0:x0 + w].astype(np.float32)
+        phi0 = rsf_amd.sdf_from_seed(np.ascontiguousarray(sb), 13.0)
+        packed[off:off + w * h * l] = oracle.run(phi0, np.ascontiguousarray(ib).astype(np.float32),
+                                                 oracle_params(_params()), 3).ravel()
+    assert np.array_equal(got, ob.reassemble(packed, shape, blocks, origins, axes, k))
