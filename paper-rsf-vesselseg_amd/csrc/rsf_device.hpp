@@ -6,6 +6,8 @@
 
 #include <utility>
 
+#include "rsf_atan.hpp"
+
 namespace rsf {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -31,52 +33,56 @@ struct Tile {
 };
 
 // ---------------------------------------------------------------- point ops
-// Arctangent in double precision from IEEE additions, multiplications and
-// divisions only: the rational approximation of S. L. Moshier's Cephes
-// library (atan.c) after its range reduction to |x| <= 0.66 (x > tan(3pi/8):
-// pi/2 - atan(1/x); x > 0.66: pi/4 + atan((x-1)/(x+1))).  Without contraction
-// every step rounds as on any IEEE machine, so the host oracle, which evaluates
-// the same sequence, gets the same bits.  Relative error below 2e-16.
-// The three reductions run as one division with selected operands, so a wave
-// whose lanes fall in different ranges takes no divergent branches: -1/ax is
-// -(1/ax) exactly, ax/1 is ax, and adding 0 to r >= 0 leaves it unchanged, so
-// the bits equal the branching form's.
-__device__ __forceinline__ double det_atan(double x) {
-#pragma clang fp contract(off)
-    const double ax = x < 0 ? -x : x;
-    const bool big = ax > 2.41421356237309504880;  // tan(3 pi / 8)
-    const bool mid = !big && !(ax <= 0.66);
-    const double num = big ? -1.0 : (mid ? ax - 1.0 : ax);
-    const double den = big ? ax : (mid ? ax + 1.0 : 1.0);
-    const double v = num / den;
-    double y = big ? 1.57079632679489661923 : (mid ? 7.85398163397448309616e-1 : 0.0);
-    const double z = v * v;
-    const double pz = (((-8.750608600031904122785e-1 * z - 1.615753718733365076637e1) * z
-                        - 7.500855792314704667340e1) * z - 1.228866684490136173410e2) * z
-                      - 6.485021904942025371773e1;
-    const double qz = ((((z + 2.485846490142306297962e1) * z + 1.650270098316988542046e2) * z
-                        + 4.328810604912902668951e2) * z + 4.853903996359136964868e2) * z
-                      + 1.945506571482613964425e2;
-    double r = v * (z * pz / qz) + v;
-    r = r + (big ? 6.123233995736765886130e-17 : (mid ? 0.5 * 6.123233995736765886130e-17 : 0.0));
-    y = y + r;
-    return x < 0 ? -y : y;
-}
-// The float arctangent of the literal path: det_atan rounded once to float,
-// i.e. the correctly rounded atanf except in (exceedingly rare) ties.
-__device__ __forceinline__ float det_atanf(float x) { return (float)det_atan((double)x); }
-
+// atanf_literal (the float arctangent of the literal path) and the unscaled
+// double division recip_nr / div_by.
 // Heaviside of CUDA_RSF/kernel.cu:258-259: float atan, double 2/pi mul-add.
 // Contraction is off inside the literal helpers so that each operation is
 // rounded on its own, as in the oracle.
 __device__ __forceinline__ float heav_literal(float phi, float eps) {
 #pragma clang fp contract(off)
-    return (float)(0.5 * (1.0 + (2.0 / kPi) * (double)det_atanf(phi / eps)));
+    return (float)(0.5 * (1.0 + (2.0 / kPi) * (double)atanf_literal(phi / eps)));
 }
 // Dirac of kernel.cu:275-276: float quotient scaled by the double 1/pi.
 __device__ __forceinline__ float dirac_literal(float phi, float eps) {
 #pragma clang fp contract(off)
     return (float)((1.0 / kPi) * (double)(eps / ((eps * eps) + (phi * phi))));
+}
+// Square root without range scaling.  The compiler's IEEE sqrtf is: inputs
+// below 2^-96 scaled by 2^32, the hardware root (within one unit), a step to
+// the neighbour s -+ 1 unit whose residual fma(-s', s, x) changes sign, the
+// scale undone, and +0 and +inf passed through.  sqrt_unscaled runs the root
+// and the step only; for x >= 2^-96, +0, +inf and NaN no scaling applies and
+// the step returns zero, infinity and NaN unchanged, so it is bit for bit
+// sqrtf there.
+__device__ __forceinline__ float sqrt_unscaled(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1), su = __int_as_float(__float_as_int(s) + 1);
+    const float r = __builtin_fmaf(-sd, s, x) <= 0.0f ? sd : s;
+    return __builtin_fmaf(-su, s, x) > 0.0f ? su : r;
+}
+// Literal unit-normal components (float)(c / ((double)M + 1e-6)) with
+// M = sqrtf(m2) the float |grad| (kernel.cu:185).  The exact form calls sqrtf
+// and the compiler's division.  The fast form uses sqrt_unscaled and one
+// reciprocal of the denominator for all components (div_by); it is bit for
+// bit the exact form unless normal_outside(m2): m2 in (0, 2^-96) (sqrt
+// scaling) or not finite (the denominator leaves div_by's range).  Finite M
+// keeps the denominator in [1e-6, 2^128] and each c is a float.
+__device__ __forceinline__ bool normal_outside(float m2) {
+    const uint32_t b = __float_as_uint(m2);
+    return b != 0u && b - 0x0f800000u > 0x7f7fffffu - 0x0f800000u;
+}
+template <int N>
+__device__ __forceinline__ void normal_literal_fast(const float (&c)[N], float m2, float (&out)[N]) {
+    const double d = (double)sqrt_unscaled(m2) + 1e-6;
+    const double r = recip_nr(d);
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = (float)div_by((double)c[i], d, r);
+}
+template <int N>
+__device__ __forceinline__ void normal_literal_exact(const float (&c)[N], float m2, float (&out)[N]) {
+    const double d = (double)sqrtf(m2) + 1e-6;
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = (float)((double)c[i] / d);
 }
 // Centred Heaviside G = H - 1/2 = atan(phi/eps)/pi (FP32).
 __device__ __forceinline__ float heav_centred(float phi, float inv_eps) {

@@ -120,6 +120,7 @@ struct Sweep2Variant {
     const void* shell_fn;  // boundary-shell kernel of a split variant (else null)
     const void* scalar_fn; // vector variants: interior kernel of blocks with unaligned rows
     const char* name;
+    const void* fix_fn;    // literal vector variant: exact recompute of the items the interior flagged
 };
 const std::vector<Sweep1Variant>& sweep1_variants();
 const std::vector<Sweep2Variant>& sweep2_variants();
@@ -133,6 +134,7 @@ struct TilePlan {
     // interior of unaligned blocks; sweep 1 z kernel of a split variant
     std::vector<Tile> tiles1, tiles2, tiles3, tiles4, tiles5;
     DevBuf d_tiles1, d_tiles2, d_tiles3, d_tiles4, d_tiles5, d_blocks;
+    DevBuf d_flags2;             // sweep 2 interior: per item and wave, fast forms out of range
     DevBuf xy;                   // split sweep 1: the two xy-blurred channels
     int64_t extent = 0;          // voxels spanned by the packed buffers
     std::vector<BlockDesc> blocks;

@@ -926,184 +926,244 @@ __global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-template <bool LIT>
-__device__ __forceinline__ float norm_comp(float c, float gx, float gy, float gz) {
+// Normal components at one point.  Literal path: EX selects the exact forms;
+// otherwise the fast forms run and `bad` records whether this lane's inputs
+// lie outside their range (the caller then recomputes with EX).
+template <bool LIT, bool EX>
+__device__ __forceinline__ float norm_comp(float c, float gx, float gy, float gz, bool& bad) {
     if constexpr (LIT) {
-        const float M = sqrtf(mag2<true>(gx, gy, gz));
-        return (float)((double)c / ((double)M + 1e-6));
+        const float m2 = mag2<true>(gx, gy, gz);
+        float n[1];
+        if constexpr (EX) {
+            normal_literal_exact<1>({c}, m2, n);
+        } else {
+            bad |= normal_outside(m2);
+            normal_literal_fast<1>({c}, m2, n);
+        }
+        return n[0];
     } else {
         const float M = __builtin_amdgcn_sqrtf(mag2<false>(gx, gy, gz));
         return c * __builtin_amdgcn_rcpf(M + 1e-6f);
     }
 }
+template <bool LIT, bool EX>
+__device__ __forceinline__ void norm_comp2(float c1, float c2, float gx, float gy, float gz, float& n1, float& n2,
+                                           bool& bad) {
+    if constexpr (LIT) {
+        const float m2 = mag2<true>(gx, gy, gz);
+        float n[2];
+        if constexpr (EX) {
+            normal_literal_exact<2>({c1, c2}, m2, n);
+        } else {
+            bad |= normal_outside(m2);
+            normal_literal_fast<2>({c1, c2}, m2, n);
+        }
+        n1 = n[0];
+        n2 = n[1];
+    } else {
+        n1 = norm_comp<false, false>(c1, gx, gy, gz, bad);
+        n2 = norm_comp<false, false>(c2, gx, gy, gz, bad);
+    }
+}
 
-template <bool LIT, int TX, int TY, int WPE>
+// FIX = 0: literal mode computes the normals with the fast forms
+// (normal_literal_fast) and records, per work item and wave, whether any lane
+// met a |grad|^2 outside their range (flags, NW = TX*TY/64 ints per item,
+// written every launch); fast mode ignores flags.  FIX = 1 (literal): the
+// exact forms, only for the waves flagged -- the fix-up launch after FIX = 0,
+// which rewrites those waves' outputs (sweep 2 reads phi and writes phi_out,
+// so a wave's part of an item can be computed again).
+template <bool LIT, int TX, int TY, int WPE, int FIX = 0>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep2_vec_kernel(
     const float* __restrict__ phi_all, const float* __restrict__ F, float* __restrict__ phi_out,
     float* __restrict__ tr_out, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles,
-    int ntiles, int zc, float dt, float nu, float mu, float eps, float inv_eps) {
+    int ntiles, int zc, float dt, float nu, float mu, float eps, float inv_eps, int* __restrict__ flags) {
+    constexpr int NW = TX * TY / 64;
+    static_assert(TX * TY % 64 == 0, "whole waves");
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
         const Tile tl = tiles[item];
         if (tl.b < 0) continue;
+        if constexpr (LIT && FIX) {
+            if (!flags[item * NW + threadIdx.x / 64]) continue;   // this wave's part only
+        }
         const BlockDesc bd = blocks[tl.b];
         const int w = bd.w, h = bd.h, l = bd.l;
         const int xend = 4 + 4 * ((w - 8) >> 2);
         const int x0 = tl.x0 + 4 * tx, y = tl.y0 + ty;
-        if (x0 >= xend || y >= h - 2) continue;
-        const int zb = tl.z0, zend = min(tl.z0 + zc, l - 2);
-        const int W = w, P = w * h;
-        const int shift = 2 * P + 2 * W + 4;
-        const float* f = phi_all + bd.off;
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc((void*)(f - shift), 0, (int)0xffffffff, 0x00020000);
-        const __amdgpu_buffer_rsrc_t rF =
-            __builtin_amdgcn_make_buffer_rsrc((void*)(F + bd.off), 0, (int)0xffffffff, 0x00020000);
-        const __amdgpu_buffer_rsrc_t rO =
-            __builtin_amdgcn_make_buffer_rsrc((void*)(phi_out + bd.off), 0, (int)0xffffffff, 0x00020000);
-        const __amdgpu_buffer_rsrc_t rT =
-            __builtin_amdgcn_make_buffer_rsrc((void*)(tr_out + bd.off), 0, (int)0xffffffff, 0x00020000);
-        // byte offset of row (y+dy, z+dz) minus 4 elements: voffset + 0..32 bytes then
-        // addresses x0-4 .. x0+4, and every sum stays non-negative
-        auto S = [&](int dy, int dz) { return (int)(4u * (uint32_t)(shift + dz * P + dy * W - 4)); };
-        const int sC = S(0, 0), sU = S(1, 0), sD = S(-1, 0), sU2 = S(2, 0), sD2 = S(-2, 0);
-        const uint32_t vstep = 4u * (uint32_t)P;
-        auto u2f = [](uint32_t u) { return __builtin_bit_cast(float, u); };
-        // centre-type row (x0-2 .. x0+5) at voxel byte offset v + row (dy, dz)
-        auto load8 = [&](float (&r)[8], uint32_t v, int so) {
-            const u32x2 a = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(v + 8), so, 0);
-            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(v + 16), so, 0);
-            const u32x2 c = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(v + 32), so, 0);
-            r[0] = u2f(a[0]); r[1] = u2f(a[1]);
-            r[2] = u2f(b[0]); r[3] = u2f(b[1]); r[4] = u2f(b[2]); r[5] = u2f(b[3]);
-            r[6] = u2f(c[0]); r[7] = u2f(c[1]);
-        };
-        // neighbour-type row (x0-1 .. x0+4)
-        auto load6 = [&](float (&r)[6], uint32_t v, int so) {
-            r[0] = u2f(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(v + 12), so, 0));
-            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(v + 16), so, 0);
-            r[1] = u2f(b[0]); r[2] = u2f(b[1]); r[3] = u2f(b[2]); r[4] = u2f(b[3]);
-            r[5] = u2f(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(v + 32), so, 0));
-        };
-        auto load4 = [&](float (&r)[4], uint32_t v, int so) {
-            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(v + 16), so, 0);
-            r[0] = u2f(b[0]); r[1] = u2f(b[1]); r[2] = u2f(b[2]); r[3] = u2f(b[3]);
-        };
-        auto loadF = [&](float (&r)[4], uint32_t v) {
-            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rF, (int)v, 0, 0);
-            r[0] = u2f(b[0]); r[1] = u2f(b[1]); r[2] = u2f(b[2]); r[3] = u2f(b[3]);
-        };
-        float C[4][8], U[4][6], D[4][6];          // phi rows, ring slot = (z' - zb) & 3
-        float Q2u[2][4], Q2d[2][4], FF[2][4];     // rows y+-2 and F of the current / next plane
-        float NZ[4][4], GZ[4][4], NX[4][4], GX[4][4];  // gradient at (x0+i, y, z')
-        const uint32_t v0 = 4u * (uint32_t)(zb * P + y * W + x0);
-        // the gradient at (x0+i, y, z') from the rows of planes z'-1, z', z'+1
-        auto zpoint = [&](int i, const float (&cm)[8], const float (&c0)[8], const float (&cp)[8],
-                          const float (&u0)[6], const float (&d0)[6], float& nx, float& gx, float& nz, float& gz) {
-            gx = (c0[i + 3] - c0[i + 1]) * 0.5f;
-            const float gy = (u0[i + 1] - d0[i + 1]) * 0.5f;
-            gz = (cp[i + 2] - cm[i + 2]) * 0.5f;
-            nx = norm_comp<LIT>(gx, gx, gy, gz);
-            nz = norm_comp<LIT>(gz, gx, gy, gz);
-        };
-        // ---- prologue: rows of planes zb-2 .. zb+2, gradients at zb-1 and zb
-        {
-            float Cmm[8];
-            load8(Cmm, v0, S(0, -2));
-            load8(C[3], v0, S(0, -1));
-            load8(C[0], v0, sC);
-            load8(C[1], v0, S(0, 1));
-            load8(C[2], v0, S(0, 2));
-            load6(U[3], v0, S(1, -1));
-            load6(D[3], v0, S(-1, -1));
-            load6(U[0], v0, sU);
-            load6(D[0], v0, sD);
-            load6(U[1], v0, S(1, 1));
-            load6(D[1], v0, S(-1, 1));
-            load4(Q2u[0], v0, sU2);
-            load4(Q2d[0], v0, sD2);
-            loadF(FF[0], v0);
+        bool bad = false;   // literal, FIX = 0: some input of this lane outside the fast forms' range
+        if (x0 < xend && y < h - 2) {
+            const int zb = tl.z0, zend = min(tl.z0 + zc, l - 2);
+            const int W = w, P = w * h;
+            const int shift = 2 * P + 2 * W + 4;
+            const float* f = phi_all + bd.off;
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc((void*)(f - shift), 0, (int)0xffffffff, 0x00020000);
+            const __amdgpu_buffer_rsrc_t rF =
+                __builtin_amdgcn_make_buffer_rsrc((void*)(F + bd.off), 0, (int)0xffffffff, 0x00020000);
+            const __amdgpu_buffer_rsrc_t rO =
+                __builtin_amdgcn_make_buffer_rsrc((void*)(phi_out + bd.off), 0, (int)0xffffffff, 0x00020000);
+            const __amdgpu_buffer_rsrc_t rT =
+                __builtin_amdgcn_make_buffer_rsrc((void*)(tr_out + bd.off), 0, (int)0xffffffff, 0x00020000);
+            // byte offset of row (y+dy, z+dz) minus 4 elements: voffset + 0..32 bytes then
+            // addresses x0-4 .. x0+4, and every sum stays non-negative
+            auto S = [&](int dy, int dz) { return (int)(4u * (uint32_t)(shift + dz * P + dy * W - 4)); };
+            const int sC = S(0, 0), sU = S(1, 0), sD = S(-1, 0), sU2 = S(2, 0), sD2 = S(-2, 0);
+            const uint32_t vstep = 4u * (uint32_t)P;
+            auto u2f = [](uint32_t u) { return __builtin_bit_cast(float, u); };
+            // centre-type row (x0-2 .. x0+5) at voxel byte offset v + row (dy, dz)
+            auto load8 = [&](float (&r)[8], uint32_t v, int so) {
+                const u32x2 a = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(v + 8), so, 0);
+                const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(v + 16), so, 0);
+                const u32x2 c = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(v + 32), so, 0);
+                r[0] = u2f(a[0]); r[1] = u2f(a[1]);
+                r[2] = u2f(b[0]); r[3] = u2f(b[1]); r[4] = u2f(b[2]); r[5] = u2f(b[3]);
+                r[6] = u2f(c[0]); r[7] = u2f(c[1]);
+            };
+            // neighbour-type row (x0-1 .. x0+4)
+            auto load6 = [&](float (&r)[6], uint32_t v, int so) {
+                r[0] = u2f(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(v + 12), so, 0));
+                const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(v + 16), so, 0);
+                r[1] = u2f(b[0]); r[2] = u2f(b[1]); r[3] = u2f(b[2]); r[4] = u2f(b[3]);
+                r[5] = u2f(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(v + 32), so, 0));
+            };
+            auto load4 = [&](float (&r)[4], uint32_t v, int so) {
+                const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(v + 16), so, 0);
+                r[0] = u2f(b[0]); r[1] = u2f(b[1]); r[2] = u2f(b[2]); r[3] = u2f(b[3]);
+            };
+            auto loadF = [&](float (&r)[4], uint32_t v) {
+                const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rF, (int)v, 0, 0);
+                r[0] = u2f(b[0]); r[1] = u2f(b[1]); r[2] = u2f(b[2]); r[3] = u2f(b[3]);
+            };
+            float C[4][8], U[4][6], D[4][6];          // phi rows, ring slot = (z' - zb) & 3
+            float Q2u[2][4], Q2d[2][4], FF[2][4];     // rows y+-2 and F of the current / next plane
+            float NZ[4][4], GZ[4][4], NX[4][4], GX[4][4];  // gradient at (x0+i, y, z')
+            const uint32_t v0 = 4u * (uint32_t)(zb * P + y * W + x0);
+            // the gradient at (x0+i, y, z') from the rows of planes z'-1, z', z'+1
+            auto zpoint = [&](auto EX_, bool& bad, int i, const float (&cm)[8], const float (&c0)[8],
+                              const float (&cp)[8], const float (&u0)[6], const float (&d0)[6], float& nx, float& gx,
+                              float& nz, float& gz) __attribute__((always_inline)) {
+                gx = (c0[i + 3] - c0[i + 1]) * 0.5f;
+                const float gy = (u0[i + 1] - d0[i + 1]) * 0.5f;
+                gz = (cp[i + 2] - cm[i + 2]) * 0.5f;
+                norm_comp2<LIT, decltype(EX_)::value>(gx, gz, gx, gy, gz, nx, nz, bad);
+            };
+            // ---- prologue: rows of planes zb-2 .. zb+2, gradients at zb-1 and zb
+            {
+                float Cmm[8];
+                load8(Cmm, v0, S(0, -2));
+                load8(C[3], v0, S(0, -1));
+                load8(C[0], v0, sC);
+                load8(C[1], v0, S(0, 1));
+                load8(C[2], v0, S(0, 2));
+                load6(U[3], v0, S(1, -1));
+                load6(D[3], v0, S(-1, -1));
+                load6(U[0], v0, sU);
+                load6(D[0], v0, sD);
+                load6(U[1], v0, S(1, 1));
+                load6(D[1], v0, S(-1, 1));
+                load4(Q2u[0], v0, sU2);
+                load4(Q2d[0], v0, sD2);
+                loadF(FF[0], v0);
+                bool unused = false;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float nx, gx;
-                zpoint(i, Cmm, C[3], C[0], U[3], D[3], nx, gx, NZ[3][i], GZ[3][i]);
-                zpoint(i, C[3], C[0], C[1], U[0], D[0], NX[0][i], GX[0][i], NZ[0][i], GZ[0][i]);
+                for (int i = 0; i < 4; ++i) {
+                    float nx, gx;
+                    zpoint(std::true_type{}, unused, i, Cmm, C[3], C[0], U[3], D[3], nx, gx, NZ[3][i], GZ[3][i]);
+                    zpoint(std::true_type{}, unused, i, C[3], C[0], C[1], U[0], D[0], NX[0][i], GX[0][i], NZ[0][i],
+                           GZ[0][i]);
+                }
+            }
+            auto step = [&](auto S_, uint32_t v, int z) __attribute__((always_inline)) {
+                constexpr int s = decltype(S_)::value;
+                constexpr int km1 = (s + 3) & 3, k0 = s & 3, kp1 = (s + 1) & 3, kp2 = (s + 2) & 3;
+                constexpr int q = s & 1, qn = (s + 1) & 1;
+                float (&Cm)[8] = C[km1];
+                float (&C0)[8] = C[k0];
+                float (&Cp)[8] = C[kp1];
+                float (&Cpp)[8] = C[kp2];
+                // the two values of plane z-1 still needed, then that slot takes plane z+3
+                const float cmL = Cm[1], cmR = Cm[6];
+                if (z + 1 < zend) {
+                    load8(C[km1], v + vstep, S(0, 2));
+                    load6(U[kp2], v + vstep, S(1, 1));
+                    load6(D[kp2], v + vstep, S(-1, 1));
+                    load4(Q2u[qn], v + vstep, sU2);
+                    load4(Q2d[qn], v + vstep, sD2);
+                    loadF(FF[qn], v + vstep);
+                }
+                // The step's arithmetic: the exact normal forms in the fix-up launch,
+                // else the fast ones, which flag a tiny or non-finite |grad|^2.
+                float p1[4], t1[4];
+                auto body = [&](auto EX_, bool& bad) __attribute__((always_inline)) {
+                    constexpr bool EX = decltype(EX_)::value;
+                    // gradients at (x0+i, y, z+1): the z+1 neighbours now, x neighbours next step
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        zpoint(EX_, bad, i, C0, Cp, Cpp, U[kp1], D[kp1], NX[kp1][i], GX[kp1][i], NZ[kp1][i],
+                               GZ[kp1][i]);
+                        if constexpr (LIT) __builtin_amdgcn_sched_barrier(0);   // one point's temporaries at a time
+                    }
+                    // x neighbours at the quad's ends: (x0-1, y, z) and (x0+4, y, z)
+                    float nxL, gxL, nxR, gxR;
+                    {
+                        gxL = (C0[2] - C0[0]) * 0.5f;
+                        const float gy = (U[k0][0] - D[k0][0]) * 0.5f, gz = (Cp[1] - cmL) * 0.5f;
+                        nxL = norm_comp<LIT, EX>(gxL, gxL, gy, gz, bad);
+                    }
+                    if constexpr (LIT) __builtin_amdgcn_sched_barrier(0);
+                    {
+                        gxR = (C0[7] - C0[5]) * 0.5f;
+                        const float gy = (U[k0][5] - D[k0][5]) * 0.5f, gz = (Cp[6] - cmR) * 0.5f;
+                        nxR = norm_comp<LIT, EX>(gxR, gxR, gy, gz, bad);
+                    }
+                    const float nxs[6] = {nxL, NX[k0][0], NX[k0][1], NX[k0][2], NX[k0][3], nxR};
+                    const float gxs[6] = {gxL, GX[k0][0], GX[k0][1], GX[k0][2], GX[k0][3], gxR};
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        // y neighbours (x0+i, y+-1, z)
+                        const float gxu = (U[k0][i + 2] - U[k0][i]) * 0.5f;
+                        const float gyu = (Q2u[q][i] - C0[i + 2]) * 0.5f;
+                        const float gzu = (U[kp1][i + 1] - U[km1][i + 1]) * 0.5f;
+                        const float nyu = norm_comp<LIT, EX>(gyu, gxu, gyu, gzu, bad);
+                        const float gxd = (D[k0][i + 2] - D[k0][i]) * 0.5f;
+                        const float gyd = (C0[i + 2] - Q2d[q][i]) * 0.5f;
+                        const float gzd = (D[kp1][i + 1] - D[km1][i + 1]) * 0.5f;
+                        const float nyd = norm_comp<LIT, EX>(gyd, gxd, gyd, gzd, bad);
+                        const float kap = half_sum3(nxs[i + 2] - nxs[i], nyu - nyd, NZ[kp1][i] - NZ[km1][i]);
+                        const float lap = half_sum3(gxs[i + 2] - gxs[i], gyu - gyd, GZ[kp1][i] - GZ[km1][i]);
+                        const float c0 = C0[i + 2];
+                        const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
+                        p1[i] = phi_update<LIT>(c0, d, FF[q][i], kap, lap, dt, nu * dt, mu);
+                        t1[i] = LIT ? heav_literal(p1[i], eps) : heav_centred(p1[i], inv_eps);
+                        if constexpr (LIT) __builtin_amdgcn_sched_barrier(0);
+                    }
+                };
+                body(std::integral_constant<bool, FIX != 0>{}, bad);
+                u32x4 po, pt;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    po[i] = __builtin_bit_cast(uint32_t, p1[i]);
+                    pt[i] = __builtin_bit_cast(uint32_t, t1[i]);
+                }
+                __builtin_amdgcn_raw_buffer_store_b128(po, rO, (int)v, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(pt, rT, (int)v, 0, 0);
+            };
+            uint32_t v = v0;
+            for (int z = zb; z < zend; z += 4) {
+                step(std::integral_constant<int, 0>{}, v, z);
+                if (z + 1 >= zend) break;
+                step(std::integral_constant<int, 1>{}, v + vstep, z + 1);
+                if (z + 2 >= zend) break;
+                step(std::integral_constant<int, 2>{}, v + 2 * vstep, z + 2);
+                if (z + 3 >= zend) break;
+                step(std::integral_constant<int, 3>{}, v + 3 * vstep, z + 3);
+                v += 4 * vstep;
             }
         }
-        auto step = [&](auto S_, uint32_t v, int z) __attribute__((always_inline)) {
-            constexpr int s = decltype(S_)::value;
-            constexpr int km1 = (s + 3) & 3, k0 = s & 3, kp1 = (s + 1) & 3, kp2 = (s + 2) & 3;
-            constexpr int q = s & 1, qn = (s + 1) & 1;
-            float (&Cm)[8] = C[km1];
-            float (&C0)[8] = C[k0];
-            float (&Cp)[8] = C[kp1];
-            float (&Cpp)[8] = C[kp2];
-            // the two values of plane z-1 still needed, then that slot takes plane z+3
-            const float cmL = Cm[1], cmR = Cm[6];
-            if (z + 1 < zend) {
-                load8(C[km1], v + vstep, S(0, 2));
-                load6(U[kp2], v + vstep, S(1, 1));
-                load6(D[kp2], v + vstep, S(-1, 1));
-                load4(Q2u[qn], v + vstep, sU2);
-                load4(Q2d[qn], v + vstep, sD2);
-                loadF(FF[qn], v + vstep);
-            }
-            // gradients at (x0+i, y, z+1): the z+1 neighbours now, x neighbours next step
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                zpoint(i, C0, Cp, Cpp, U[kp1], D[kp1], NX[kp1][i], GX[kp1][i], NZ[kp1][i], GZ[kp1][i]);
-            // x neighbours at the quad's ends: (x0-1, y, z) and (x0+4, y, z)
-            float nxL, gxL, nxR, gxR;
-            {
-                gxL = (C0[2] - C0[0]) * 0.5f;
-                const float gy = (U[k0][0] - D[k0][0]) * 0.5f, gz = (Cp[1] - cmL) * 0.5f;
-                nxL = norm_comp<LIT>(gxL, gxL, gy, gz);
-            }
-            {
-                gxR = (C0[7] - C0[5]) * 0.5f;
-                const float gy = (U[k0][5] - D[k0][5]) * 0.5f, gz = (Cp[6] - cmR) * 0.5f;
-                nxR = norm_comp<LIT>(gxR, gxR, gy, gz);
-            }
-            const float nxs[6] = {nxL, NX[k0][0], NX[k0][1], NX[k0][2], NX[k0][3], nxR};
-            const float gxs[6] = {gxL, GX[k0][0], GX[k0][1], GX[k0][2], GX[k0][3], gxR};
-            float p1[4], t1[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                // y neighbours (x0+i, y+-1, z)
-                const float gxu = (U[k0][i + 2] - U[k0][i]) * 0.5f;
-                const float gyu = (Q2u[q][i] - C0[i + 2]) * 0.5f;
-                const float gzu = (U[kp1][i + 1] - U[km1][i + 1]) * 0.5f;
-                const float nyu = norm_comp<LIT>(gyu, gxu, gyu, gzu);
-                const float gxd = (D[k0][i + 2] - D[k0][i]) * 0.5f;
-                const float gyd = (C0[i + 2] - Q2d[q][i]) * 0.5f;
-                const float gzd = (D[kp1][i + 1] - D[km1][i + 1]) * 0.5f;
-                const float nyd = norm_comp<LIT>(gyd, gxd, gyd, gzd);
-                const float kap = half_sum3(nxs[i + 2] - nxs[i], nyu - nyd, NZ[kp1][i] - NZ[km1][i]);
-                const float lap = half_sum3(gxs[i + 2] - gxs[i], gyu - gyd, GZ[kp1][i] - GZ[km1][i]);
-                const float c0 = C0[i + 2];
-                const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
-                p1[i] = phi_update<LIT>(c0, d, FF[q][i], kap, lap, dt, nu * dt, mu);
-                t1[i] = LIT ? heav_literal(p1[i], eps) : heav_centred(p1[i], inv_eps);
-            }
-            u32x4 po, pt;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                po[i] = __builtin_bit_cast(uint32_t, p1[i]);
-                pt[i] = __builtin_bit_cast(uint32_t, t1[i]);
-            }
-            __builtin_amdgcn_raw_buffer_store_b128(po, rO, (int)v, 0, 0);
-            __builtin_amdgcn_raw_buffer_store_b128(pt, rT, (int)v, 0, 0);
-        };
-        uint32_t v = v0;
-        for (int z = zb; z < zend; z += 4) {
-            step(std::integral_constant<int, 0>{}, v, z);
-            if (z + 1 >= zend) break;
-            step(std::integral_constant<int, 1>{}, v + vstep, z + 1);
-            if (z + 2 >= zend) break;
-            step(std::integral_constant<int, 2>{}, v + 2 * vstep, z + 2);
-            if (z + 3 >= zend) break;
-            step(std::integral_constant<int, 3>{}, v + 3 * vstep, z + 3);
-            v += 4 * vstep;
+        if constexpr (LIT && !FIX) {
+            const bool wbad = __builtin_amdgcn_ballot_w64(bad) != 0;
+            if (threadIdx.x % 64 == 0) flags[item * NW + threadIdx.x / 64] = wbad ? 1 : 0;
         }
     }
 }
@@ -1409,11 +1469,12 @@ Sweep2Variant s2s(const char* name) {
                          (const void*)&sweep2_shell_kernel<LIT>, nullptr, name};
 }
 // vector interior for blocks with 16-byte aligned rows, the 64x4 scalar interior for the others
+// (literal: the fast-form interior plus its exact fix-up)
 template <bool LIT, int TX, int TY, int WPE>
 Sweep2Variant s2q(const char* name) {
-    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 4, (const void*)&sweep2_vec_kernel<LIT, TX, TY, WPE>,
+    return Sweep2Variant{LIT ? 1 : 0, TX, TY, 4, (const void*)&sweep2_vec_kernel<LIT, TX, TY, WPE, 0>,
                          (const void*)&sweep2_shell_kernel<LIT>, (const void*)&sweep2_inner_kernel<LIT, 64, 4>,
-                         name};
+                         name, LIT ? (const void*)&sweep2_vec_kernel<LIT, TX, TY, WPE, 1> : nullptr};
 }
 
 // split sweep 1: xy kernel with TX x TY tiles, z kernel with TXC x TYC columns
@@ -1619,6 +1680,12 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     RSF_TRY(tp.d_blocks.ensure(blocks.size() * sizeof(BlockDesc)));
     RSF_HIP(hipMemcpy(tp.d_tiles1.p, tp.tiles1.data(), tp.tiles1.size() * sizeof(Tile), hipMemcpyHostToDevice));
     RSF_HIP(hipMemcpy(tp.d_tiles2.p, tp.tiles2.data(), tp.tiles2.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    if (V2[s2].fix_fn && !tp.tiles2.empty()) {
+        // every launch of the interior rewrites its flags; zero them for a clean start
+        const size_t nf = tp.tiles2.size() * (size_t)(V2[s2].tx * V2[s2].ty / 64) * sizeof(int);
+        RSF_TRY(tp.d_flags2.ensure(nf));
+        RSF_HIP(hipMemset(tp.d_flags2.p, 0, nf));
+    }
     if (!tp.tiles3.empty()) {
         RSF_TRY(tp.d_tiles3.ensure(tp.tiles3.size() * sizeof(Tile)));
         RSF_HIP(hipMemcpy(tp.d_tiles3.p, tp.tiles3.data(), tp.tiles3.size() * sizeof(Tile), hipMemcpyHostToDevice));
@@ -1676,8 +1743,9 @@ int launch_sweep2(const TilePlan& tp, const SweepArgs& a, const rsf_params* p, h
     void* args[] = {(void*)&a.phi,  (void*)&a.F,     (void*)&a.phi_out, (void*)&a.tr_out,
                     (void*)&tp.d_blocks.p, (void*)&tp.d_tiles2.p, (void*)&ntiles, (void*)&tp.zc2,
                     (void*)&p->dt,  (void*)&p->nu,   (void*)&p->mu,     (void*)&p->epsilon,
-                    (void*)&inv_eps};
+                    (void*)&inv_eps, (void*)&tp.d_flags2.p};   // (kernels without flags ignore the last)
     if (ntiles > 0) RSF_HIP(hipLaunchKernel(v.fn, dim3(tp.grid2), dim3(v.tx * v.ty), args, 0, s));
+    if (ntiles > 0 && v.fix_fn) RSF_HIP(hipLaunchKernel(v.fix_fn, dim3(tp.grid2), dim3(v.tx * v.ty), args, 0, s));
     if (v.scalar_fn && tp.grid4 > 0) {
         const int n4 = (int)tp.tiles4.size();
         void* a4[] = {(void*)&a.phi,  (void*)&a.F,     (void*)&a.phi_out, (void*)&a.tr_out,

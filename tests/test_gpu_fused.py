@@ -331,3 +331,26 @@ def test_batched_at_other_tap_counts(ctx, mode):
             ref = oracle.run(phi[off:off + n].reshape(shp), img[off:off + n].reshape(shp).astype(np.float32),
                              oracle_params(p), 4)
             assert np.array_equal(A[off:off + n].reshape(shp), ref)
+
+
+@pytest.mark.parametrize("shape", [(36, 30, 52), (30, 33, 37)])
+def test_literal_tiny_and_huge_gradients_take_the_exact_forms(ctx, shape):
+    """Gradients below 2^-48 (|grad|^2 under the square root's scaling range)
+    and above 2^64 (|grad|^2 overflows to infinity) are outside the fast
+    forms' range: the literal interior flags those work items and the exact
+    fix-up pass recomputes them.  One step equals the oracle bit for bit."""
+    import torch
+
+    img, phi0, _ = scene(shape, tubes=5, seed=91)
+    l = shape[0]
+    phi = phi0.astype(np.float64)
+    phi[: l // 3] *= 1e-20            # tiny values: tiny differences
+    phi[2 * l // 3:] *= 1e21          # huge values: |grad|^2 = inf in float
+    phi = phi.astype(np.float32)
+    assert np.isfinite(phi).all()
+    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, mode=rsf_amd.MODE_LITERAL, iterations=1)
+    d_out = torch.empty(shape, dtype=torch.float32, device="cuda")
+    ctx.step(to_dev(img), to_dev(phi), d_out, p)
+    torch.cuda.synchronize()
+    ref = oracle.step(phi, img.astype(np.float32), oracle_params(p))
+    assert np.array_equal(d_out.cpu().numpy(), ref)
