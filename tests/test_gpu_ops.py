@@ -127,3 +127,25 @@ def test_mirror_rejects_bad_arguments():
         rsf_amd.check(rsf_amd.lib().rsf_heaviside_cal(_p(a), None, *_dims()))
     with pytest.raises(rsf_amd.RSFError):
         rsf_amd.check(rsf_amd.lib().rsf_heaviside_cal(_p(a), _p(a), 0, 3, 3))
+
+
+def test_mirror_heaviside_wide_range():
+    """The device arctangent (degree-9 polynomial, unscaled division) on ~4M
+    inputs spread over magnitudes 2^-30 .. 2^30, and the runs of consecutive
+    floats where the arctangent is flattest, against the oracle bit for bit."""
+    rng = np.random.default_rng(7)
+    shape = (64, 256, 256)
+    n = int(np.prod(shape))
+    mag = np.exp2(rng.uniform(-30, 30, n))
+    phi = (mag * rng.choice([-1.0, 1.0], n)).astype(np.float32)
+    runs = [np.float32(62921456.0) * 0.2, np.float32(0.00035211083) * 0.2, np.float32(1.0278158) * 0.2]
+    k = 0
+    for c in runs:
+        b = np.array([c], np.float32).view(np.int32)[0]
+        phi[k:k + 4096] = (b + np.arange(-2048, 2048, dtype=np.int32)).view(np.float32)
+        k += 4096
+    phi[k:k + 6] = [0.0, -0.0, np.inf, -np.inf, 1e-45, -3e38]
+    phi = phi.reshape(shape)
+    out = np.empty_like(phi)
+    rsf_amd.check(rsf_amd.lib().rsf_heaviside_cal(_p(out), _p(phi), *_dims(shape)))
+    assert np.array_equal(out, oracle.heaviside(phi))
