@@ -364,6 +364,20 @@ def cpu_baseline(img, phi0, budget_s):
         iters += extra
     n = phi0.size
     log(f"cpu oracle: {iters} iterations of {n / 1e6:.0f} Mvoxel in {t1:.1f} s on {threads} threads")
+    # one thread, on a bounded crop of the same volume (one iteration of at most 256^3)
+    c = tuple(min(256, d) for d in phi0.shape)
+    cphi = np.ascontiguousarray(phi0[:c[0], :c[1], :c[2]])
+    cI = np.ascontiguousarray(I[:c[0], :c[1], :c[2]])
+    t0 = time.perf_counter()
+    oracle.run(cphi, cI, p, 1, 1)
+    t_one = time.perf_counter() - t0
+    log(f"cpu oracle, 1 thread: 1 iteration of {c[2]}x{c[1]}x{c[0]} in {t_one:.1f} s")
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": round(n * iters / t1 / 1e9, 5),
         "unit": "Gvoxel-iterations/s",
@@ -371,6 +385,9 @@ def cpu_baseline(img, phi0, budget_s):
         "kind": "port",
         "sample": f"{iters} iteration(s) of the same {round(n ** (1 / 3))}^3 workload (oracle: unfused FP32 "
                   f"restatement of CUDA_RSF/main.cpp:234-300, OpenMP)",
+        "one_thread": {"value": round(cphi.size / t_one / 1e9, 5), "unit": "Gvoxel-iterations/s",
+                       "sample": f"1 iteration of a {c[2]}x{c[1]}x{c[0]} crop of the same workload"},
+        "host": {"nproc": os.cpu_count(), "model_name": model},
     }
 
 
