@@ -253,12 +253,18 @@ int rsf_initial_phi(const uint8_t* image, float* phi, int64_t w, int64_t h, int6
  * seed - shift > 0 form the region; phi is the signed Euclidean distance to
  * its boundary, POSITIVE inside that region (the sign of seed - shift). */
 int rsf_sdf_from_seed(const float* seed, float* phi, int64_t w, int64_t h, int64_t l, float shift);
-/* rsf_sdf_from_seed on device buffers (seed uint8 or float32, by seed_dtype),
- * bit-identical to the host form: an exact separable squared EDT, x rows by
- * forward/backward scans, y and z by outward scans.  The volume diagonal must
- * be below 4096 voxels. */
+/* rsf_sdf_from_seed on device buffers (seed uint8 or float32, by seed_dtype):
+ * an exact separable squared EDT, x rows by forward/backward scans, y and z by
+ * outward scans.  Below a volume diagonal of 4096 voxels the squared distances
+ * are exact in float and the result is the host form's bit for bit; larger
+ * volumes carry them in double (8 + 8 bytes of workspace per voxel) and round
+ * each distance once to float. */
 int rsf_sdf_from_seed_device(rsf_context* ctx, const void* seed, int seed_dtype, float* phi, int64_t w, int64_t h,
                              int64_t l, float shift);
+/* rsf_sdf_from_seed_device on host buffers (float32 seed): upload, compute on
+ * the device, download (main.cpp:174-183, the rsf CLI's block modes). */
+int rsf_sdf_from_seed_host(rsf_context* ctx, const float* seed, float* phi, int64_t w, int64_t h, int64_t l,
+                           float shift);
 
 /* ------------------------------------------ mirrors of the reference wrappers
  * Same names (with an rsf_ prefix), argument order and meaning as the host

@@ -112,25 +112,33 @@ std::string load_image(const std::string& path, Image* im) {
     return "";
 }
 
-// phi0 = sdf(seed - 13) (main.cpp:174-183), or the array itself with --phi0.
-std::string load_phi0(const std::string& path, bool is_phi0, const std::vector<int64_t>& shape,
-                      std::vector<float>* phi) {
+// The seed (or, with --phi0, the initial level set) as float32, shape-checked.
+std::string load_seed(const std::string& path, const std::vector<int64_t>& shape, std::vector<float>* seed) {
     rsf::NpyArray s;
     std::string err = rsf::npy_load(path, &s);
     if (!err.empty()) return err;
     if (s.shape != shape) return path + ": initial level set and image shapes differ";
-    std::vector<float> seed;
-    err = rsf::npy_to_f32(s, &seed);
-    if (!err.empty()) return err;
-    if (is_phi0) {
-        phi->swap(seed);
-        return "";
-    }
-    phi->resize(seed.size());
-    if (rsf_sdf_from_seed(seed.data(), phi->data(), shape[2], shape[1], shape[0], 13.0f) != RSF_OK)
+    return rsf::npy_to_f32(s, seed);
+}
+
+// phi0 = sdf(seed - 13) (main.cpp:174-183) on the device, in place; with --phi0
+// the array is phi0 itself.
+std::string make_phi0(rsf_context* ctx, bool is_phi0, const std::vector<int64_t>& shape, std::vector<float>* phi) {
+    if (is_phi0) return "";
+    std::vector<float> seed(phi->size());
+    seed.swap(*phi);
+    if (rsf_sdf_from_seed_host(ctx, seed.data(), phi->data(), shape[2], shape[1], shape[0], 13.0f) != RSF_OK)
         return rsf_last_error();
     return "";
 }
+
+// Destroys the context on every exit path.
+struct ContextGuard {
+    rsf_context* c = nullptr;
+    ~ContextGuard() {
+        if (c) rsf_context_destroy(c);
+    }
+};
 
 void print_iterations(rsf_context* ctx, int T) {
     float ms = 0.f;
@@ -180,11 +188,17 @@ int run_batch(const std::string& manifest, bool is_phi0, const rsf_params& p, in
         items.emplace_back();
         Item& it = items.back();
         std::string err = load_image(resolve(a), &it.im);
-        if (err.empty()) err = load_phi0(resolve(b), is_phi0, it.im.arr.shape, &it.phi);
+        if (err.empty()) err = load_seed(resolve(b), it.im.arr.shape, &it.phi);
         if (!err.empty()) return fail(err);
         it.out = resolve(c);
     }
     if (items.empty()) return fail("manifest " + manifest + " lists no blocks");
+    ContextGuard g;
+    if (rsf_context_create(device, nullptr, &g.c) != RSF_OK) return fail(rsf_last_error());
+    for (Item& it : items) {
+        const std::string err = make_phi0(g.c, is_phi0, it.im.arr.shape, &it.phi);
+        if (!err.empty()) return fail(err);
+    }
     const bool u8 = std::all_of(items.begin(), items.end(), [](const Item& it) { return it.im.u8; });
     std::vector<rsf_block> blocks;
     int64_t total = 0;
@@ -209,13 +223,10 @@ int run_batch(const std::string& manifest, bool is_phi0, const rsf_params& p, in
             std::copy(it.im.f32.begin(), it.im.f32.end(), imgf.begin() + off);
         }
     }
-    rsf_context* ctx = nullptr;
-    if (rsf_context_create(device, nullptr, &ctx) != RSF_OK) return fail(rsf_last_error());
-    const int st = rsf_run_batched_host(ctx, blocks.data(), (int)blocks.size(),
+    const int st = rsf_run_batched_host(g.c, blocks.data(), (int)blocks.size(),
                                         u8 ? (const void*)img8.data() : (const void*)imgf.data(),
                                         u8 ? RSF_DTYPE_U8 : RSF_DTYPE_F32, phi.data(), &p);
-    if (st == RSF_OK) print_iterations(ctx, p.iterations);
-    rsf_context_destroy(ctx);
+    if (st == RSF_OK) print_iterations(g.c, p.iterations);
     if (st != RSF_OK) return fail(std::string(rsf_status_string(st)) + ": " + rsf_last_error());
     for (size_t i = 0; i < items.size(); ++i) {
         const std::string err = rsf::npy_save_f32(items[i].out, phi.data() + blocks[i].offset, items[i].im.arr.shape);
@@ -313,31 +324,30 @@ int main(int argc, char** argv) {
         for (int k = 0; k < p.iterations; ++k) std::cout << "It takes " << ms[k] << " ms to calculate " << std::endl;
         return 0;
     }
+    // --block: the whole volume cut into overlapping blocks, run together, blended
+    int64_t bs[3];
+    if (!parse_dims(val[idx("block")], bs)) return fail("--block must be N or W,H,L (positive integers)");
+    float ov = 32.f;
+    if (seen[idx("overlap")] && (!parse_float(val[idx("overlap")], &ov) || ov < 0 || ov != (float)(int)ov))
+        return fail("--overlap must be a non-negative integer");
     Image im;
     std::string err = load_image(val[idx("image")], &im);
     if (!err.empty()) return fail(err);
     const std::vector<int64_t>& shape = im.arr.shape;
     const int64_t l = shape[0], h = shape[1], w = shape[2];
     std::vector<float> phi;
-    err = load_phi0(val[idx(use_phi0 ? "phi0" : "binary")], use_phi0, shape, &phi);
+    err = load_seed(val[idx(use_phi0 ? "phi0" : "binary")], shape, &phi);
+    if (!err.empty()) return fail(err);
+    ContextGuard g;
+    if (rsf_context_create(device, nullptr, &g.c) != RSF_OK) return fail(rsf_last_error());
+    err = make_phi0(g.c, use_phi0, shape, &phi);
     if (!err.empty()) return fail(err);
     const void* image_ptr = im.u8 ? (const void*)im.arr.data.data() : (const void*)im.f32.data();
     const int dtype = im.u8 ? RSF_DTYPE_U8 : RSF_DTYPE_F32;
-    if (seen[idx("block")]) {
-        int64_t bs[3];
-        if (!parse_dims(val[idx("block")], bs)) return fail("--block must be N or W,H,L (positive integers)");
-        float ov = 32.f;
-        if (seen[idx("overlap")] && (!parse_float(val[idx("overlap")], &ov) || ov < 0 || ov != (float)(int)ov))
-            return fail("--overlap must be a non-negative integer");
-        rsf_context* ctx = nullptr;
-        if (rsf_context_create(device, nullptr, &ctx) != RSF_OK) return fail(rsf_last_error());
-        const int st = rsf_run_volume_blocks_host(ctx, image_ptr, dtype, phi.data(), w, h, l, bs[0], bs[1], bs[2],
-                                                  (int64_t)ov, &p);
-        if (st == RSF_OK) print_iterations(ctx, p.iterations);
-        rsf_context_destroy(ctx);
-        if (st != RSF_OK) return fail(std::string(rsf_status_string(st)) + ": " + rsf_last_error());
-        err = rsf::npy_save_f32(out_path, phi.data(), shape);
-        return err.empty() ? 0 : fail(err);
-    }
-    return 0;
+    const int st = rsf_run_volume_blocks_host(g.c, image_ptr, dtype, phi.data(), w, h, l, bs[0], bs[1], bs[2],
+                                              (int64_t)ov, &p);
+    if (st == RSF_OK) print_iterations(g.c, p.iterations);
+    if (st != RSF_OK) return fail(std::string(rsf_status_string(st)) + ": " + rsf_last_error());
+    err = rsf::npy_save_f32(out_path, phi.data(), shape);
+    return err.empty() ? 0 : fail(err);
 }

@@ -245,16 +245,6 @@ int rsf_run_npy(rsf_context* c, const char* image_path, const char* seed_path, i
     RSF_TRY(npy_load_device(s, image_path, img.p, dtype, (int64_t)img.bytes, &shape));
     if (seed_is_phi0) {
         RSF_TRY(npy_load_device(s, seed_path, phi.p, RSF_DTYPE_F32, (int64_t)phi.bytes, &shape));
-    } else if (!((float)w * (float)w + (float)h * (float)h + (float)l * (float)l < 16777216.0f)) {
-        // beyond the device SDF's exact range: phi0 = sdf(seed - 13) on the host
-        NpyArray sa;
-        std::vector<float> seedf, phi0((size_t)n);
-        std::string err = npy_load(seed_path, &sa);
-        if (err.empty()) err = npy_to_f32(sa, &seedf);
-        if (!err.empty()) return set_error(RSF_ERR_IO, "%s", err.c_str());
-        RSF_TRY(rsf_sdf_from_seed(seedf.data(), phi0.data(), w, h, l, 13.0f));
-        RSF_HIP(hipMemcpyAsync(phi.p, phi0.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
-        RSF_HIP(hipStreamSynchronize(s));
     } else {
         // phi0 = sdf(seed - 13), main.cpp:174-183
         DevBuf seed;

@@ -16,12 +16,15 @@
 //     (|G(c')| if G(c') has v's sign else 0) + (c - c')^2,
 //   scanned outward from c and stopped once (c - c')^2 >= the best so far.
 // A wave holds 64 consecutive x, so every scan step is one coalesced load.
-// The volume diagonal is kept below 4096 voxels, so every squared distance is
-// an integer below 2^24 (or kInf) and float arithmetic on them is exact.
+// Below a volume diagonal of 4096 voxels every squared distance is an integer
+// below 2^24 (or kInf), float arithmetic on them is exact, and the result is
+// the host form's bit for bit.  Larger volumes carry the squared distances in
+// double (exact below 2^53) and round the final distance once to float.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "rsf_internal.hpp"
@@ -29,7 +32,14 @@
 namespace rsf {
 namespace {
 
-constexpr float kInf = 1e20f;  // "no such voxel on this line"; kInf + r^2 rounds back to kInf
+// "no such voxel on this line"; kInf + r^2 rounds back to kInf
+template <typename D>
+__device__ __forceinline__ D sdf_inf() { return sizeof(D) == 4 ? (D)1e20f : (D)1e300; }
+template <typename D>
+__device__ __forceinline__ D dmin(D a, D b) {
+    if constexpr (sizeof(D) == 4) return fminf(a, b);
+    else return fmin(a, b);
+}
 
 template <typename T>
 __device__ __forceinline__ bool is_inside(const T* seed, int64_t i, float shift) {
@@ -38,8 +48,8 @@ __device__ __forceinline__ bool is_inside(const T* seed, int64_t i, float shift)
 
 // Pass x.  flags[z] collects bit 0 (plane z has an inside voxel) and bit 1
 // (an outside voxel).
-template <typename T>
-__global__ void k_sdf_rows(const T* __restrict__ seed, float shift, float* __restrict__ G, int64_t w, int64_t h,
+template <typename T, typename D>
+__global__ void k_sdf_rows(const T* __restrict__ seed, float shift, D* __restrict__ G, int64_t w, int64_t h,
                            int64_t rows, int* __restrict__ flags) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= rows) return;
@@ -48,7 +58,7 @@ __global__ void k_sdf_rows(const T* __restrict__ seed, float shift, float* __res
     for (int64_t x = 0; x < w; ++x) {
         const bool ins = is_inside(seed, base + x, shift);
         const int64_t other = ins ? last_out : last_in;
-        G[base + x] = other >= 0 ? (float)(x - other) : -1.0f;
+        G[base + x] = other >= 0 ? (D)(x - other) : (D)-1;
         if (ins) last_in = x;
         else last_out = x;
     }
@@ -57,9 +67,9 @@ __global__ void k_sdf_rows(const T* __restrict__ seed, float shift, float* __res
     for (int64_t x = w - 1; x >= 0; --x) {
         const bool ins = is_inside(seed, base + x, shift);
         const int64_t other = ins ? next_out : next_in;
-        float d = G[base + x];
-        if (other >= 0 && (d < 0.0f || (float)(other - x) < d)) d = (float)(other - x);
-        const float q = d < 0.0f ? kInf : d * d;
+        D d = G[base + x];
+        if (other >= 0 && (d < (D)0 || (D)(other - x) < d)) d = (D)(other - x);
+        const D q = d < (D)0 ? sdf_inf<D>() : d * d;
         G[base + x] = ins ? -q : q;
         if (ins) next_in = x;
         else next_out = x;
@@ -71,34 +81,40 @@ __global__ void k_sdf_rows(const T* __restrict__ seed, float shift, float* __res
 // Pass y (FINAL = false, stride w, out = G') or pass z (FINAL = true, stride
 // w*h, out = phi).  plane_flags (pass y): a plane without both kinds of voxel
 // keeps +-kInf.  scan = false: the volume lacks one kind, every line stays kInf.
-template <bool FINAL>
-__global__ void k_sdf_pass(const float* __restrict__ in, float* __restrict__ out, int64_t w, int64_t h, int64_t n,
-                           int64_t stride, int64_t len, const int* __restrict__ plane_flags, bool scan, float cap) {
+template <bool FINAL, typename D>
+__global__ void k_sdf_pass(const D* __restrict__ in, std::conditional_t<FINAL, float, D>* __restrict__ out, int64_t w,
+                           int64_t h, int64_t n, int64_t stride, int64_t len, const int* __restrict__ plane_flags,
+                           bool scan, D cap) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
-        const float g = in[v];
-        const bool neg = g < 0.0f;
-        float best = fabsf(g);
+        const D g = in[v];
+        const bool neg = g < (D)0;
+        D best = g < (D)0 ? -g : g;
         const int64_t c = FINAL ? v / (w * h) : (v / w) % h;
         const bool go = scan && (FINAL || plane_flags[v / (w * h)] == 3);
         if (go) {
             for (int64_t r = 1; r < len; ++r) {
-                const float rr = (float)(r * r);
+                const D rr = (D)(r * r);
                 if (rr >= best) break;
                 const bool lo = c - r >= 0, hi = c + r < len;
                 if (!lo && !hi) break;
                 if (lo) {
-                    const float o = in[v - r * stride];
-                    best = fminf(best, (o < 0.0f) == neg ? fabsf(o) + rr : rr);
+                    const D o = in[v - r * stride];
+                    best = dmin(best, (o < (D)0) == neg ? (o < (D)0 ? -o : o) + rr : rr);
                 }
                 if (hi) {
-                    const float o = in[v + r * stride];
-                    best = fminf(best, (o < 0.0f) == neg ? fabsf(o) + rr : rr);
+                    const D o = in[v + r * stride];
+                    best = dmin(best, (o < (D)0) == neg ? (o < (D)0 ? -o : o) + rr : rr);
                 }
             }
         }
         if constexpr (FINAL) {
-            const float a = fminf(sqrtf(best), cap) - 0.5f;
-            out[v] = neg ? a : -a;
+            if constexpr (sizeof(D) == 4) {
+                const float a = fminf(sqrtf(best), cap) - 0.5f;
+                out[v] = neg ? a : -a;
+            } else {
+                const double a = fmin(sqrt(best), cap) - 0.5;
+                out[v] = (float)(neg ? a : -a);
+            }
         } else {
             out[v] = neg ? -best : best;
         }
@@ -112,6 +128,41 @@ unsigned grid_for(int64_t n, int64_t cap) { return (unsigned)std::max<int64_t>(1
 
 using namespace rsf;
 
+namespace rsf {
+namespace {
+
+// The three passes with squared distances of type D; G0 (n elements of D) may
+// alias phi when D is float.
+template <typename D>
+int sdf_passes(hipStream_t s, const void* seed, int seed_dtype, float shift, float* phi, D* G0, D* G1, int64_t w,
+               int64_t h, int64_t l, D cap, int* flags) {
+    const int64_t n = w * h * l, rows = h * l;
+    RSF_HIP(hipMemsetAsync(flags, 0, (size_t)l * sizeof(int), s));
+    if (seed_dtype == RSF_DTYPE_U8)
+        hipLaunchKernelGGL((k_sdf_rows<uint8_t, D>), dim3(grid_for(rows, INT32_MAX)), dim3(256), 0, s,
+                           (const uint8_t*)seed, shift, G0, w, h, rows, flags);
+    else
+        hipLaunchKernelGGL((k_sdf_rows<float, D>), dim3(grid_for(rows, INT32_MAX)), dim3(256), 0, s,
+                           (const float*)seed, shift, G0, w, h, rows, flags);
+    RSF_HIP(hipGetLastError());
+    std::vector<int> pf((size_t)l);
+    RSF_HIP(hipMemcpyAsync(pf.data(), flags, (size_t)l * sizeof(int), hipMemcpyDeviceToHost, s));
+    RSF_HIP(hipStreamSynchronize(s));
+    int all = 0;
+    for (int f : pf) all |= f;
+    const bool scan = all == 3;  // otherwise no distance is finite
+    const unsigned g = grid_for(n, 1 << 20);
+    hipLaunchKernelGGL((k_sdf_pass<false, D>), dim3(g), dim3(256), 0, s, G0, G1, w, h, n, w, h, flags, scan, cap);
+    RSF_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_sdf_pass<true, D>), dim3(g), dim3(256), 0, s, G1, phi, w, h, n, w * h, l, flags, scan, cap);
+    RSF_HIP(hipGetLastError());
+    RSF_HIP(hipStreamSynchronize(s));
+    return RSF_OK;
+}
+
+}  // namespace
+}  // namespace rsf
+
 extern "C" int rsf_sdf_from_seed_device(rsf_context* c, const void* seed, int seed_dtype, float* phi, int64_t w,
                                         int64_t h, int64_t l, float shift) {
     hipStream_t s;
@@ -121,37 +172,39 @@ extern "C" int rsf_sdf_from_seed_device(rsf_context* c, const void* seed, int se
         return set_error(RSF_ERR_INVALID_ARGUMENT, "seed dtype must be RSF_DTYPE_U8 or RSF_DTYPE_F32");
     if (w <= 0 || h <= 0 || l <= 0 || w * h * l >= (int64_t(1) << 40))
         return set_error(RSF_ERR_INVALID_ARGUMENT, "bad dims");
+    const int64_t n = w * h * l;
+    DevBuf flags;
+    RSF_TRY(flags.ensure((size_t)l * sizeof(int)));
     // the volume diagonal, evaluated as the host form does
     const float diag2 = (float)w * (float)w + (float)h * (float)h + (float)l * (float)l;
-    if (!(diag2 < 16777216.0f))
-        return set_error(RSF_ERR_INVALID_ARGUMENT,
-                         "volume diagonal must be below 4096 voxels (squared distances exact in float)");
-    const float cap = std::sqrt(diag2);
-    const int64_t n = w * h * l, rows = h * l;
-    DevBuf tmp, flags;
-    RSF_TRY(tmp.ensure((size_t)n * sizeof(float)));
-    RSF_TRY(flags.ensure((size_t)l * sizeof(int)));
-    RSF_HIP(hipMemsetAsync(flags.p, 0, (size_t)l * sizeof(int), s));
-    if (seed_dtype == RSF_DTYPE_U8)
-        hipLaunchKernelGGL(k_sdf_rows<uint8_t>, dim3(grid_for(rows, INT32_MAX)), dim3(256), 0, s,
-                           (const uint8_t*)seed, shift, phi, w, h, rows, flags.as<int>());
-    else
-        hipLaunchKernelGGL(k_sdf_rows<float>, dim3(grid_for(rows, INT32_MAX)), dim3(256), 0, s, (const float*)seed,
-                           shift, phi, w, h, rows, flags.as<int>());
-    RSF_HIP(hipGetLastError());
-    std::vector<int> pf((size_t)l);
-    RSF_HIP(hipMemcpyAsync(pf.data(), flags.p, (size_t)l * sizeof(int), hipMemcpyDeviceToHost, s));
-    RSF_HIP(hipStreamSynchronize(s));
-    int all = 0;
-    for (int f : pf) all |= f;
-    const bool scan = all == 3;  // otherwise no distance is finite
-    const unsigned g = grid_for(n, 1 << 20);
-    hipLaunchKernelGGL(k_sdf_pass<false>, dim3(g), dim3(256), 0, s, phi, tmp.as<float>(), w, h, n, w, h,
-                       flags.as<int>(), scan, cap);
-    RSF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sdf_pass<true>, dim3(g), dim3(256), 0, s, tmp.as<float>(), phi, w, h, n, w * h, l,
-                       flags.as<int>(), scan, cap);
-    RSF_HIP(hipGetLastError());
+    if (diag2 < 16777216.0f) {
+        DevBuf tmp;
+        RSF_TRY(tmp.ensure((size_t)n * sizeof(float)));
+        return sdf_passes<float>(s, seed, seed_dtype, shift, phi, phi, tmp.as<float>(), w, h, l, std::sqrt(diag2),
+                                 flags.as<int>());
+    }
+    DevBuf g0, g1;
+    RSF_TRY(g0.ensure((size_t)n * sizeof(double)));
+    RSF_TRY(g1.ensure((size_t)n * sizeof(double)));
+    const double cap = std::sqrt((double)w * (double)w + (double)h * (double)h + (double)l * (double)l);
+    return sdf_passes<double>(s, seed, seed_dtype, shift, phi, g0.as<double>(), g1.as<double>(), w, h, l, cap,
+                              flags.as<int>());
+}
+
+extern "C" int rsf_sdf_from_seed_host(rsf_context* c, const float* seed, float* phi, int64_t w, int64_t h, int64_t l,
+                                      float shift) {
+    hipStream_t s;
+    RSF_TRY(context_stream(c, &s));
+    if (!seed || !phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (w <= 0 || h <= 0 || l <= 0 || w * h * l >= (int64_t(1) << 40))
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "bad dims");
+    const size_t bytes = (size_t)(w * h * l) * sizeof(float);
+    DevBuf ds, dp;
+    RSF_TRY(ds.ensure(bytes));
+    RSF_TRY(dp.ensure(bytes));
+    RSF_HIP(hipMemcpyAsync(ds.p, seed, bytes, hipMemcpyHostToDevice, s));
+    RSF_TRY(rsf_sdf_from_seed_device(c, ds.p, RSF_DTYPE_F32, dp.as<float>(), w, h, l, shift));
+    RSF_HIP(hipMemcpyAsync(phi, dp.p, bytes, hipMemcpyDeviceToHost, s));
     RSF_HIP(hipStreamSynchronize(s));
     return RSF_OK;
 }

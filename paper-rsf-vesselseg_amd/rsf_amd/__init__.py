@@ -117,6 +117,7 @@ _SIGS = {
     "rsf_synth_tubes": (_I, [_P, _P, _I64, _I64, _I64, _I, _F, _F, _F, _F, C.c_uint64]),
     "rsf_initial_phi": (_I, [_P, _P, _I64, _I64, _I64, _F, _F]),
     "rsf_sdf_from_seed": (_I, [_P, _P, _I64, _I64, _I64, _F]),
+    "rsf_sdf_from_seed_host": (_I, [_P, _P, _P, _I64, _I64, _I64, _F]),
     "rsf_sdf_from_seed_device": (_I, [_P, _P, _I, _P, _I64, _I64, _I64, _F]),
     "rsf_heaviside_cal": (_I, [_P, _P, _I, _I, _I]),
     "rsf_deri_heaviside_cal": (_I, [_P, _P, _I, _I, _I]),

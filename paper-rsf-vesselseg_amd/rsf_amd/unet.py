@@ -15,7 +15,7 @@ What follows the reference exactly:
   including the repeated edge tiles that its clamping produces;
 * `vote`: count / total in float64 with every voxel's tile outputs summed in
   the loop's order (run_model.py:86-91), then probability >= 0.5
-  (run_model.py:29-35).  The reference unpacks that function's results into
+  (run_model.py:29-35), on the device.  The reference unpacks that function's results into
   swapped names (run_model.py:91): the array it saves as "processed" is the
   probability and the one it saves as "probability" is the 0/1 map.
 
@@ -139,62 +139,26 @@ def predict_tiles(model, x: torch.Tensor, slices, batch: int = 64) -> torch.Tens
     return out
 
 
-def _cover(starts, ends, n, dev):
-    """Per coordinate of an axis: the tiles covering it in loop order (index, offset),
-    padded with index -1."""
-    starts, ends = np.asarray(starts), np.asarray(ends)
-    c = np.arange(n)[:, None]
-    hit = (c >= starts[None, :]) & (c < ends[None, :])
-    m = int(hit.sum(1).max())
-    idx = np.full((n, m), -1, np.int64)
-    off = np.zeros((n, m), np.int64)
-    for q in range(n):
-        t = np.nonzero(hit[q])[0]
-        idx[q, :len(t)] = t
-        off[q, :len(t)] = q - starts[t]
-    return torch.from_numpy(idx).to(dev), torch.from_numpy(off).to(dev)
-
-
 def vote(pred: torch.Tensor, shape, slices):
-    """count / total of run_model.py:86-91 from the tile outputs `pred` (T, 16, 64, 64).
+    """count / total of run_model.py:86-91 from the tile outputs `pred` (T, 16, 64, 64)
+    on a HIP device (rsf_tile_vote_device).
 
     Every voxel sums its covering tiles' outputs in float64 in the reference's
     loop order, so the result is the reference's bit for bit given the same tile
     outputs.  Returns (probability float64, map bool) on pred's device; voxels no
-    tile covers get 0.  Device tensors go through the HIP kernel
-    (rsf_tile_vote_device); host tensors through an exact gather per tile
-    combination in torch."""
+    tile covers get 0."""
+    if pred.device.type != "cuda":
+        raise ValueError("the tile vote runs on the device: pass the tile outputs as a HIP tensor")
     l, h, w = shape
     dev = pred.device
-    nz = math.ceil(l / NUM_PICS)
     ny = math.ceil(h / (HEIGHT // STEP_RATIO))
     nx = math.ceil(w / (WIDTH // STEP_RATIO))
-    if dev.type == "cuda":
-        ctx = _vote_context(dev)
-        prob = torch.empty(shape, dtype=torch.float64, device=dev)
-        mask = torch.empty(shape, dtype=torch.uint8, device=dev)
-        starts = ([s[0] for s in slices[::ny * nx]], [s[2] for s in slices[:ny * nx:nx]], [s[4] for s in slices[:nx]])
-        ctx.tile_vote(pred.contiguous(), (NUM_PICS, HEIGHT, WIDTH), starts, shape, prob, mask)
-        return prob, mask.bool()
-    zi, zo = _cover([s[0] for s in slices[::ny * nx]], [s[1] for s in slices[::ny * nx]], l, dev)
-    yi, yo = _cover([s[2] for s in slices[:ny * nx:nx]], [s[3] for s in slices[:ny * nx:nx]], h, dev)
-    xi, xo = _cover([s[4] for s in slices[:nx]], [s[5] for s in slices[:nx]], w, dev)
-    count = torch.zeros(shape, dtype=torch.float64, device=dev)
-    total = torch.zeros(shape, dtype=torch.float64, device=dev)
-    flat = pred.reshape(-1)
-    for a in range(zi.shape[1]):
-        for b in range(yi.shape[1]):
-            for c in range(xi.shape[1]):
-                ti = (zi[:, a, None, None] * ny + yi[None, :, b, None]) * nx + xi[None, None, :, c]
-                ok = (zi[:, a, None, None] >= 0) & (yi[None, :, b, None] >= 0) & (xi[None, None, :, c] >= 0)
-                lin = ((ti * NUM_PICS + zo[:, a, None, None]) * HEIGHT + yo[None, :, b, None]) * WIDTH + \
-                    xo[None, None, :, c]
-                v = flat[torch.where(ok, lin, torch.zeros_like(lin))].double()
-                count += torch.where(ok, v, torch.zeros_like(v))
-                total += ok.double()
-    prob = torch.where(total > 0, count / torch.where(total > 0, total, torch.ones_like(total)),
-                       torch.zeros_like(count))
-    return prob, prob >= 0.5
+    ctx = _vote_context(dev)
+    prob = torch.empty(shape, dtype=torch.float64, device=dev)
+    mask = torch.empty(shape, dtype=torch.uint8, device=dev)
+    starts = ([s[0] for s in slices[::ny * nx]], [s[2] for s in slices[:ny * nx:nx]], [s[4] for s in slices[:nx]])
+    ctx.tile_vote(pred.contiguous(), (NUM_PICS, HEIGHT, WIDTH), starts, shape, prob, mask)
+    return prob, mask.bool()
 
 
 _contexts: dict = {}
@@ -210,8 +174,10 @@ def _vote_context(dev):
 
 
 def segment(model, volume: np.ndarray, mode: str | None = None, device="cuda", batch: int = 64):
-    """Normalize, run the tiles through the model on `device`, vote.  Returns the
-    float64 probability and the bool map as device tensors."""
+    """Normalize, run the tiles through the model on the HIP `device`, vote there.
+    Returns the float64 probability and the bool map as device tensors."""
+    if torch.device(device).type != "cuda":
+        raise ValueError("segment runs on a HIP device")
     x = torch.from_numpy(normalize(volume, mode)).to(device)
     sl = tile_slices(volume.shape)
     pred = predict_tiles(model, x, sl, batch)

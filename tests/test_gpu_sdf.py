@@ -1,5 +1,6 @@
 """phi0 = sdf(seed - 13) (CUDA_RSF/main.cpp:174-183) on the device: bit-identical
-to the host form, which the CPU suite checks against scipy's exact EDT."""
+to the host form (which the CPU suite checks against scipy's exact EDT) below a
+4096-voxel diagonal, and equal to scipy's EDT rounded once beyond it."""
 import numpy as np
 import pytest
 
@@ -70,7 +71,41 @@ def test_device_sdf_rejects_bad_input(ctx):
         ctx.sdf_from_seed(torch.zeros((4, 4, 4), dtype=torch.int32, device="cuda"), d)
     seed = torch.zeros((4, 4, 4), device="cuda")
     lib = rsf_amd.lib()
-    # an unknown dtype code, and a diagonal of 4096 voxels or more: rejected before any access
+    # an unknown dtype code, an empty dimension and NULL buffers: rejected before any access
     assert lib.rsf_sdf_from_seed_device(ctx.handle, seed.data_ptr(), 7, d.data_ptr(), 4, 4, 4, 13.0) != 0
     assert lib.rsf_sdf_from_seed_device(ctx.handle, seed.data_ptr(), rsf_amd.DTYPE_F32, d.data_ptr(),
-                                        4096, 1, 1, 13.0) != 0
+                                        0, 4, 4, 13.0) != 0
+    assert lib.rsf_sdf_from_seed_device(ctx.handle, None, rsf_amd.DTYPE_F32, d.data_ptr(), 4, 4, 4, 13.0) != 0
+    assert lib.rsf_sdf_from_seed_host(ctx.handle, None, None, 4, 4, 4, 13.0) != 0
+
+
+def _edt_phi(seed, shift=13.0):
+    """The exact field from scipy's float64 EDT: positive inside {seed - shift > 0},
+    distances capped at the volume diagonal, minus 1/2, rounded once to float."""
+    from scipy import ndimage
+
+    inside = (seed.astype(np.float32) - np.float32(shift)) > 0
+    l, h, w = seed.shape
+    cap = np.sqrt(float(w) * w + float(h) * h + float(l) * l)
+    d_in = ndimage.distance_transform_edt(inside)
+    d_out = ndimage.distance_transform_edt(~inside)
+    return np.where(inside, np.minimum(d_in, cap) - 0.5, -(np.minimum(d_out, cap) - 0.5)).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape", [(4, 40, 4100), (4100, 6, 5), (3, 2900, 2900)])
+def test_device_sdf_beyond_4096_diagonal_is_exact(ctx, shape):
+    """Volumes whose diagonal reaches 4096 voxels carry the squared distances in
+    double: the field equals scipy's exact EDT rounded once to float."""
+    import torch
+
+    rng = np.random.default_rng(sum(shape))
+    seed = np.zeros(shape, np.float32)
+    for _ in range(5):
+        c = [int(rng.integers(0, n)) for n in shape]
+        seed[tuple(slice(max(0, ci - 2), ci + 3) for ci in c)] = 255.0
+    l, h, w = shape
+    assert w * w + h * h + l * l >= 4096 ** 2
+    d = torch.empty(shape, dtype=torch.float32, device="cuda")
+    ctx.sdf_from_seed(to_dev(seed), d, 13.0)
+    torch.cuda.synchronize()
+    assert np.array_equal(d.cpu().numpy(), _edt_phi(seed))

@@ -1,6 +1,7 @@
-"""The U-Net seed producer on the GPU: the vote bit-identical to the CPU one,
-the model on the GPU against the CPU, the seed hand-off into the device SDF
-and the RSF run, and gradients from an RSF loss back into the U-Net."""
+"""The U-Net seed producer on the GPU: the device vote bit-identical to the
+oracle's, the whole segment() on the GPU against the model on the CPU voted by
+the oracle, the seed hand-off into the device SDF and the RSF run, and
+gradients from an RSF loss back into the U-Net."""
 import numpy as np
 import pytest
 
@@ -19,29 +20,34 @@ def ctx():
     c.close()
 
 
-def test_vote_on_device_equals_cpu():
+@pytest.mark.parametrize("shape", [(40, 100, 130), (17, 65, 97), (16, 64, 64)])
+def test_vote_on_device_equals_oracle(shape):
     import torch
 
+    from oracle import unet_vote
     from rsf_amd import unet
 
-    shape = (40, 100, 130)
     sl = unet.tile_slices(shape)
     pred = torch.rand((len(sl), unet.NUM_PICS, unet.HEIGHT, unet.WIDTH), generator=torch.Generator().manual_seed(1))
-    pc, mc = unet.vote(pred, shape, sl)
+    pc, mc = unet_vote.tile_vote(pred, shape, sl)
     pg, mg = unet.vote(pred.cuda(), shape, sl)
     assert torch.equal(pg.cpu(), pc) and torch.equal(mg.cpu(), mc)
 
 
-def test_segment_on_device_matches_cpu():
+def test_segment_on_device_matches_cpu_model_and_oracle_vote():
     import torch
 
+    from oracle import unet_vote
     from rsf_amd import unet
 
     torch.manual_seed(0)
     model = unet.UNet3D().eval()
     img, _, _ = scene((32, 96, 96), tubes=5, seed=61)
-    pc, _ = unet.segment(model, img, device="cpu", batch=8)
+    sl = unet.tile_slices(img.shape)
+    pred = unet.predict_tiles(model, torch.from_numpy(unet.normalize(img)), sl, batch=8)
+    pc, _ = unet_vote.tile_vote(pred, img.shape, sl)
     pg, _ = unet.segment(model.cuda(), img, device="cuda", batch=32)
+    # the model runs in FP32 on both sides (different convolution kernels)
     assert float((pg.cpu() - pc).abs().max()) < 1e-4
 
 

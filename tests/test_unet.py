@@ -1,12 +1,14 @@
-"""U-Net seed producer on the CPU: the tile list and the vote against a literal
-restatement of Unet/run_model.py's loop, the model's shape contract, and the
-identity-model check of the whole segment() path."""
+"""U-Net seed producer on the CPU: the tile list and the oracle's vote against a
+literal restatement of Unet/run_model.py's loop, the model's shape contract,
+the identity-model check of tiling + prediction + vote, and the product vote's
+refusal of host tensors (it runs on the device only)."""
 import math
 
 import numpy as np
 import pytest
 import torch
 
+from oracle import unet_vote
 from rsf_amd import unet
 
 
@@ -37,6 +39,11 @@ def reference_vote(pred, shape, slices):
     return prob, prob >= 0.5
 
 
+class _Identity(torch.nn.Module):
+    def forward(self, x):
+        return x
+
+
 SHAPES = [(16, 64, 64), (32, 128, 128), (40, 100, 130), (17, 65, 97)]
 
 
@@ -54,15 +61,24 @@ def test_tile_slices_reject_volumes_smaller_than_a_tile():
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-def test_vote_equals_reference_bit_for_bit(shape):
+def test_oracle_vote_equals_reference_bit_for_bit(shape):
     sl = unet.tile_slices(shape)
     rng = np.random.default_rng(len(sl))
     pred = rng.random((len(sl), unet.NUM_PICS, unet.HEIGHT, unet.WIDTH), dtype=np.float32)
     pred[0, 0, 0, :4] = [1e-30, 3e-12, 0.5, 1.0]     # values whose float64 sums round
-    prob, mask = unet.vote(torch.from_numpy(pred), shape, sl)
+    prob, mask = unet_vote.tile_vote(torch.from_numpy(pred), shape, sl)
     rp, rm = reference_vote(pred, shape, sl)
     assert np.array_equal(prob.numpy(), rp)
     assert np.array_equal(mask.numpy(), rm)
+
+
+def test_product_vote_and_segment_refuse_the_host():
+    shape = (16, 64, 64)
+    sl = unet.tile_slices(shape)
+    with pytest.raises(ValueError, match="device"):
+        unet.vote(torch.zeros((len(sl), unet.NUM_PICS, unet.HEIGHT, unet.WIDTH)), shape, sl)
+    with pytest.raises(ValueError, match="HIP device"):
+        unet.segment(_Identity(), np.zeros(shape, np.uint8), device="cpu")
 
 
 def test_model_shape_and_range():
@@ -81,16 +97,13 @@ def test_load_model_reads_a_tensor_state_dict(tmp_path):
         assert torch.equal(a, b)
 
 
-class _Identity(torch.nn.Module):
-    def forward(self, x):
-        return x
-
-
-def test_segment_with_an_identity_model_returns_the_normalized_volume():
+def test_tiles_of_an_identity_model_vote_back_to_the_normalized_volume():
     rng = np.random.default_rng(4)
     vol = rng.integers(0, 256, (20, 70, 90), dtype=np.uint8)
-    prob, mask = unet.segment(_Identity(), vol, device="cpu", batch=7)
     x = unet.normalize(vol)
+    sl = unet.tile_slices(vol.shape)
+    pred = unet.predict_tiles(_Identity(), torch.from_numpy(x), sl, batch=7)
+    prob, mask = unet_vote.tile_vote(pred, vol.shape, sl)
     # every voxel averages n copies of float32(x): the float64 mean is float32(x) exactly
     assert np.array_equal(prob.numpy(), x.astype(np.float32).astype(np.float64))
     assert np.array_equal(mask.numpy(), x.astype(np.float32) >= 0.5)
