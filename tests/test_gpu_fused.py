@@ -354,3 +354,32 @@ def test_literal_tiny_and_huge_gradients_take_the_exact_forms(ctx, shape):
     torch.cuda.synchronize()
     ref = oracle.step(phi, img.astype(np.float32), oracle_params(p))
     assert np.array_equal(d_out.cpu().numpy(), ref)
+
+
+def test_literal_fixup_in_a_batched_launch(ctx):
+    """The fix-up pass in a batched launch: of three uneven blocks, one has tiny
+    phi values throughout, one huge ones in part, one ordinary ones; one step of
+    each equals the oracle bit for bit."""
+    shapes = [(30, 34, 44), (26, 40, 52), (33, 30, 37)]
+    scales = [1e-20, None, 1.0]
+    blocks, total = rsf_amd.pack_blocks(shapes)
+    im = np.zeros(total, np.uint8)
+    ph = np.zeros(total, np.float32)
+    refs = []
+    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, mode=rsf_amd.MODE_LITERAL, iterations=1)
+    for (off, w, h, l), shp, sc in zip(blocks, shapes, scales):
+        bi, bp, _ = scene(shp, tubes=3, seed=93 + w)
+        b = bp.astype(np.float64)
+        if sc is None:
+            b[: l // 2] *= 1e21
+        else:
+            b *= sc
+        b = b.astype(np.float32)
+        im[off:off + bi.size] = bi.ravel()
+        ph[off:off + b.size] = b.ravel()
+        refs.append(oracle.step(b, bi.astype(np.float32), oracle_params(p)))
+    d_ph = to_dev(ph)
+    ctx.run_batched(blocks, to_dev(im), d_ph, p)
+    got = d_ph.cpu().numpy()
+    for (off, w, h, l), ref in zip(blocks, refs):
+        assert np.array_equal(got[off:off + w * h * l].reshape(l, h, w), ref)
