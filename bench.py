@@ -265,7 +265,7 @@ def io_run(rsf_amd, torch, ctx):
 def unet_run(rsf_amd, torch, ctx):
     """U-Net seed producer (rsf_amd.unet, random weights: the reference ships none) on a
     128x512x512 tube phantom: 16x64x64 tiles at stride 32, fp32, 64 tiles per forward,
-    then the float64 vote; the host normalization is timed apart."""
+    then the float64 vote; the (device) normalization is timed apart."""
     from rsf_amd import unet
 
     shape = (128, 512, 512)
@@ -273,7 +273,7 @@ def unet_run(rsf_amd, torch, ctx):
     torch.manual_seed(0)
     model = unet.UNet3D().cuda().eval()
     t0 = time.perf_counter()
-    x = torch.from_numpy(unet.normalize(img)).cuda()
+    x = unet.normalize_device(img)
     torch.cuda.synchronize()
     t_norm = time.perf_counter() - t0
     sl = unet.tile_slices(shape)

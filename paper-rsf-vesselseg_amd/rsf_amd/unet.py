@@ -105,6 +105,26 @@ def normalize(volume: np.ndarray, mode: str | None = None) -> np.ndarray:
     return (x - np.min(x)) / (np.max(x) - np.min(x))
 
 
+def normalize_device(volume: np.ndarray, mode: str | None = None, device="cuda") -> torch.Tensor:
+    """`normalize` with the result on `device`.  Mode None (min-max scaling, the
+    default) runs there with numpy's arithmetic -- integer volumes subtract the
+    minimum in their own type and divide in float64, float volumes stay in their
+    type -- so the values are normalize()'s bit for bit; the scipy modes run on
+    the host as in the reference and are uploaded."""
+    if mode is not None or volume.dtype not in (np.uint8, np.uint16, np.float32, np.float64):
+        # (signed integers: numpy's x - min may wrap; kept on the host to match it)
+        return torch.from_numpy(normalize(volume, mode)).to(device)
+    if volume.ndim != 3:
+        raise ValueError("volume must be (Z, Y, X)")
+    t = torch.from_numpy(np.ascontiguousarray(volume)).to(device)
+    if not t.dtype.is_floating_point:
+        t = t.long()   # unsigned: x - min is exact in the source type and in int64 alike
+    mn, mx = t.min(), t.max()
+    if t.dtype.is_floating_point:
+        return (t - mn) / (mx - mn)
+    return (t - mn).double() / (mx - mn).double()
+
+
 def tile_slices(shape):
     """(z0, z1, y0, y1, x0, x1) of every tile of a (l, h, w) volume in the order
     run_model.py:82-85 visits them.  Each axis must hold at least one tile."""
@@ -178,7 +198,7 @@ def segment(model, volume: np.ndarray, mode: str | None = None, device="cuda", b
     Returns the float64 probability and the bool map as device tensors."""
     if torch.device(device).type != "cuda":
         raise ValueError("segment runs on a HIP device")
-    x = torch.from_numpy(normalize(volume, mode)).to(device)
+    x = normalize_device(volume, mode, device)
     sl = tile_slices(volume.shape)
     pred = predict_tiles(model, x, sl, batch)
     return vote(pred, volume.shape, sl)

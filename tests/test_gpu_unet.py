@@ -91,3 +91,18 @@ def test_rsf_loss_backpropagates_into_the_unet():
     grads = [p.grad for p in model.parameters()]
     assert all(g is not None and bool(torch.isfinite(g).all()) for g in grads)
     assert sum(float(g.abs().sum()) for g in grads) > 0
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32, np.uint16, np.int16])
+def test_normalize_on_device_equals_host(dtype):
+    import torch
+
+    from rsf_amd import unet
+
+    rng = np.random.default_rng(3)
+    vol = (rng.random((20, 70, 90)) * 250 + 3).astype(dtype)
+    got = unet.normalize_device(vol)
+    ref = unet.normalize(vol)
+    assert got.dtype == torch.from_numpy(ref).dtype
+    assert torch.equal(got.cpu(), torch.from_numpy(ref))
+    assert torch.equal(unet.normalize_device(vol, "blur").cpu(), torch.from_numpy(unet.normalize(vol, "blur")))
