@@ -144,7 +144,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} is missing: run `make -C {PKG_DIR}` (or __graft_entry__.build())")
+            raise ImportError(f"{LIB_PATH} is missing: run `make -C {PKG_DIR}`")
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)

@@ -28,7 +28,7 @@ def oracle_params(p: rsf_amd.Params):
 
 
 def one_step_gate(got, ref, rel=1e-4):
-    """North-star gate after one step: max|dphi| <= rel * max|phi|."""
+    """The one-step parity gate: max|dphi| <= rel * max|phi|."""
     err = float(np.abs(got.astype(np.float64) - ref).max())
     tol = rel * float(np.abs(ref).max())
     return err, tol

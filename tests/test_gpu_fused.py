@@ -1,5 +1,5 @@
 """GPU parity of the fused engine (sweeps, graph-captured loop, batched
-blocks) against the oracle, with the north-star gates: after one step
+blocks) against the oracle, with the parity gates: after one step
 max|dphi| <= 1e-4 * max|phi|; after a run the sign masks differ on <= 1e-4
 of the voxels."""
 import numpy as np
@@ -94,7 +94,7 @@ def test_run_matches_oracle_mask(ctx, mode):
     ref = oracle.run(phi0, img.astype(np.float32), oracle_params(p), 40)
     assert mask_disagreement(got, ref) <= 1e-4
     # away from the front the two agree closely; single voxels at the front can
-    # drift after many steps (the gate above is the north-star criterion)
+    # drift after many steps (the gate above is the acceptance criterion)
     diff = np.abs(got - ref)
     assert float(np.median(diff)) <= 1e-5 * float(np.abs(ref).max())
 
