@@ -369,10 +369,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     constexpr int WIN = (RX + P - 1 + 3) & ~3;
     static_assert(NXI % NT == 0 && NYI % NT == 0 && TY % R == 0, "work items must be multiples of the threads");
     __shared__ __align__(16) float sIn[CIN][RH][RWP];
-    // x-pass output: channel planes CS floats apart, CS = 32 (mod 64), so the two
-    // halves of a wave in the y pass (same rows, neighbouring channels) read
-    // disjoint LDS banks
-    constexpr int CS0 = RH * TXP, CS = CS0 + (((32 - CS0 % 64) % 64) + 64) % 64;
+    // x-pass output: channel planes CS floats apart, CS = TX (mod 64), so the
+    // groups of TX lanes of a wave in the y pass (same rows, neighbouring
+    // channels) read disjoint LDS banks
+    constexpr int CS0 = RH * TXP, CS = CS0 + (((TX % 64 - CS0 % 64) % 64) + 64) % 64;
     __shared__ __align__(16) float sX[CH * CS];
     const int tid = threadIdx.x;
     const uint8_t* img8 = (const uint8_t*)img;
@@ -1504,6 +1504,9 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, false, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
         s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
+        // literal default: tall narrow tiles (x pass over 128 rows for 108 outputs)
+        s1s<21, true, 16, 108, 256, 9, 2, 256, 1, 3, true, 8>("p21_lit_hi_16x108x8"),
+        s1s<21, true, 16, 108, 256, 9, 2, 256, 1, 3, true>("p21_lit_hi_16x108r9"),
         s1s<21, true, 32, 60, 256, 10, 2, 256, 1, 3, true>("p21_lit_hi_32x60r10"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true, 8>("p21_lit_hi_32x44x8"),
