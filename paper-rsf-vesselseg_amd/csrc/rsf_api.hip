@@ -250,8 +250,14 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
     };
     // no compiled variant for this P (or mode): the caller takes the unfused path
     *zc1 = *s1 >= 0 ? chunk(sweep1_variants()[*s1].tx, sweep1_variants()[*s1].ty, 2, 4 * d.P) : std::min(lmax, 128);
-    // split sweep 1: the xy kernel has no z state, short chunks balance its work items
-    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) *zc1 = std::min(lmax, 32);
+    // split sweep 1: the xy kernel has no z state, short chunks balance its work items;
+    // from 2^28 voxels on, 64-plane chunks measured faster (16x108 literal tiles at
+    // 1024x1024x768: 15.1 ms against 17.8 with 32 planes; at 512^3 32 planes win)
+    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) {
+        int64_t vox = 0;
+        for (const auto& b : blocks) vox += (int64_t)b.w * b.h * b.l;
+        *zc1 = std::min(lmax, vox >= (int64_t(1) << 28) ? 64 : 32);
+    }
     *zc2 = *s2 >= 0 ? std::min(chunk(sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, sweep2_variants()[*s2].ty,
                                      4, 16), 64)
                     : std::min(lmax, 64);

@@ -27,6 +27,7 @@
 
 #include "npy.hpp"
 #include "rsf.h"
+#include "rsf_cli.h"
 
 namespace {
 
@@ -237,7 +238,9 @@ int run_batch(const std::string& manifest, bool is_phi0, const rsf_params& p, in
 
 }  // namespace
 
-int main(int argc, char** argv) {
+namespace {
+
+int cli(int argc, char** argv) {
     std::vector<std::string> val(kNumOpts);
     std::vector<bool> seen(kNumOpts, false);
     for (int a = 1; a < argc; ++a) {
@@ -351,3 +354,18 @@ int main(int argc, char** argv) {
     err = rsf::npy_save_f32(out_path, phi.data(), shape);
     return err.empty() ? 0 : fail(err);
 }
+
+}  // namespace
+
+// The whole command line as a C entry point: the executable's main, and a way to
+// run the same code in a process that already uses the GPU (no child process).
+extern "C" int rsf_cli_main(int argc, char** argv) {
+    const int rc = cli(argc, argv);
+    std::cout.flush();
+    std::cerr.flush();
+    return rc;
+}
+
+#ifndef RSF_CLI_NO_MAIN
+int main(int argc, char** argv) { return rsf_cli_main(argc, argv); }
+#endif

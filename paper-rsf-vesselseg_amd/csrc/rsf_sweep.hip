@@ -1504,9 +1504,10 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, false, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
         s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
-        // literal default: tall narrow tiles (x pass over 128 rows for 108 outputs)
-        s1s<21, true, 16, 108, 256, 9, 2, 256, 1, 3, true, 8>("p21_lit_hi_16x108x8"),
+        // literal default: tall narrow tiles (x pass over 128 rows for 108 outputs); the
+        // 8-output x items are faster at 512^3 but spill, and lose on larger volumes
         s1s<21, true, 16, 108, 256, 9, 2, 256, 1, 3, true>("p21_lit_hi_16x108r9"),
+        s1s<21, true, 16, 108, 256, 9, 2, 256, 1, 3, true, 8>("p21_lit_hi_16x108x8"),
         s1s<21, true, 32, 60, 256, 10, 2, 256, 1, 3, true>("p21_lit_hi_32x60r10"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true, 8>("p21_lit_hi_32x44x8"),

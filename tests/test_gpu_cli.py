@@ -1,14 +1,14 @@
-"""The rsf executable on the GPU: the plain run, the --block pipeline and the
+"""The rsf command line on the GPU: the plain run, the --block pipeline and the
 --batch manifest equal the library calls they stand for, and literal mode
-equals the oracle."""
-import subprocess
-
+equals the oracle.  The command line runs in this process (rsf_cli_main, the
+code of the rsf executable): the test process already uses the GPU and starts
+no child."""
 import numpy as np
 import pytest
 
 import oracle
 import rsf_amd
-from gpu_util import oracle_params, scene, to_dev
+from gpu_util import oracle_params, run_cli, scene, to_dev
 from oracle import blocks as ob
 
 pytestmark = pytest.mark.gpu
@@ -18,7 +18,7 @@ ARGS = ["--sigma", "1", "--t", "3", "--dt", "0.1", "--meu", "1", "--e", "1", "--
 
 
 def _cli(*args, cwd=None):
-    return subprocess.run([rsf_amd.CLI_PATH, *args], capture_output=True, text=True, cwd=cwd, timeout=120)
+    return run_cli(*args, cwd=cwd)
 
 
 def _params():

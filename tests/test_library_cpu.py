@@ -31,6 +31,16 @@ def test_library_exports_every_declared_symbol():
     assert sorted(set(names) - set(rsf_amd.EXPORTED)) == []
 
 
+def test_cli_library_exports_its_entry_point():
+    import ctypes
+
+    text = open(os.path.join(ROOT, "include", "rsf_cli.h")).read()
+    assert "int rsf_cli_main(int argc, char** argv);" in text
+    rsf_amd.lib()
+    L = ctypes.CDLL(os.path.join(rsf_amd.PKG_DIR, "librsf_cli.so"))
+    assert hasattr(L, "rsf_cli_main")
+
+
 def test_abi_version_and_status_strings():
     L = rsf_amd.lib()
     assert L.rsf_abi_version() == 1
