@@ -409,7 +409,9 @@ def main():
     ap.add_argument("--size", type=int, default=512, help="cube side of the plain workload")
     ap.add_argument("--mode", choices=["fast", "literal"], default="literal")
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-mode line of the literal run")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the plain runs too (the plain drop-in runs the library defaults)")
+    ap.add_argument("--no-tune", action="store_true", help="the default for the plain runs (kept for old commands)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=300)
@@ -452,7 +454,7 @@ def main():
     mode = rsf_amd.MODE_FAST if args.mode == "fast" else rsf_amd.MODE_LITERAL
     ctx = rsf_amd.Context(0)
     img, phi0 = phantom(rsf_amd, args.size)
-    r = plain_run(rsf_amd, torch, ctx, img, phi0, args.steps, args.warmup, mode, not args.no_tune)
+    r = plain_run(rsf_amd, torch, ctx, img, phi0, args.steps, args.warmup, mode, args.tune and not args.no_tune)
     N, K = r["vox"], args.steps
     value = N * K / r["wall"] / 1e9
     dom = "sweep1" if r["ms1"] >= r["ms2"] else "sweep2"
@@ -505,7 +507,8 @@ def main():
     if not args.no_batched:
         line["batched"] = batched_run(rsf_amd, torch, ctx, args.batched_steps, mode)
     if mode == rsf_amd.MODE_LITERAL and not args.no_fast:
-        rf = plain_run(rsf_amd, torch, ctx, img, phi0, args.steps, args.warmup, rsf_amd.MODE_FAST, not args.no_tune)
+        rf = plain_run(rsf_amd, torch, ctx, img, phi0, args.steps, args.warmup, rsf_amd.MODE_FAST,
+                       args.tune and not args.no_tune)
         line["fast_mode"] = {
             "value": round(N * K / rf["wall"] / 1e9, 3),
             "ms_per_step": round(rf["wall"] * 1e3 / K, 4),

@@ -241,15 +241,6 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
     const int lmax = max_dim_l(blocks);
     int64_t vox = 0;
     for (const auto& b : blocks) vox += (int64_t)b.w * b.h * b.l;
-    // literal 16x108 tiles: 8 x-pass outputs per item are faster below 2^28 voxels
-    // (512^3: sweep 1 2.39 ms against 2.43-2.51), 4 beyond (1024x1024x768: 15.1
-    // against 16.8 ms)
-    if (literal && vox < (int64_t(1) << 28) && *s1 >= 0 &&
-        std::strcmp(sweep1_variants()[*s1].name, "p21_lit_hi_16x108r9") == 0) {
-        const auto& V = sweep1_variants();
-        for (size_t k = 0; k < V.size(); ++k)
-            if (std::strcmp(V[k].name, "p21_lit_hi_16x108x8") == 0) *s1 = (int)k;
-    }
     // Longest z-chunks that still give every CU a few tiles: a sweep-1 chunk
     // re-reads P-1 halo planes, a sweep-2 chunk restarts its register pipeline.
     auto chunk = [&](int tx, int ty, int per_cu, int zmin) {
@@ -262,9 +253,10 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
     // no compiled variant for this P (or mode): the caller takes the unfused path
     *zc1 = *s1 >= 0 ? chunk(sweep1_variants()[*s1].tx, sweep1_variants()[*s1].ty, 2, 4 * d.P) : std::min(lmax, 128);
     // split sweep 1: the xy kernel has no z state, short chunks balance its work items;
-    // from 2^28 voxels on, 64-plane chunks measured faster (16x108 literal tiles at
-    // 1024x1024x768: 15.1 ms against 17.8 with 32 planes; at 512^3 32 planes win)
-    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) *zc1 = std::min(lmax, vox >= (int64_t(1) << 28) ? 64 : 32);
+    // from 2^28 voxels on, longer chunks measured faster (32x60 literal tiles at
+    // 1024x1024x768: 15.0 ms with 128 planes, 15.4 with 64, 16.4 with 32; at 512^3
+    // 32 planes win)
+    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) *zc1 = std::min(lmax, vox >= (int64_t(1) << 28) ? 128 : 32);
     *zc2 = *s2 >= 0 ? std::min(chunk(sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, sweep2_variants()[*s2].ty,
                                      4, 16), 64)
                     : std::min(lmax, 64);

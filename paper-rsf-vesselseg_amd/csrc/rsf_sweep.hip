@@ -341,11 +341,17 @@ __device__ __forceinline__ void sweep1_channels(float i, float t, float (&c)[LIT
 // outputs to LDS (RX = 4 per work item), the y pass R outputs per work item
 // from R+P-1 LDS reads.  The work items of both passes are exact multiples of
 // the thread count.  Channel ch of the output goes to xy + ch * stride.
+//
+// IMG: the image dtype fixed at compile time (1 uint8, 0 float32; -1 takes the
+// img_u8 argument), so that a uint8 image's prefetch keeps one packed word per
+// four voxels in registers.
 template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, bool HI = false, int RX = 4, int UX = 1,
-          int UY = 1>
+          int UY = 1, int IMG = -1>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_kernel(
-    const float* __restrict__ tr, const void* __restrict__ img, int img_u8, float* __restrict__ xy, int64_t stride,
-    const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc, Taps taps) {
+    const float* __restrict__ tr, const void* __restrict__ img, int img_u8_arg, float* __restrict__ xy,
+    int64_t stride, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc,
+    Taps taps) {
+    const int img_u8 = IMG >= 0 ? IMG : img_u8_arg;
     // HI (literal mode): LDS holds H and I, and each x-pass item forms a pair
     // of channels from them -- (I*H, H) or (I*(1-H), 1-H) -- before blurring,
     // which halves the LDS input tile.
@@ -1494,8 +1500,11 @@ Sweep1Variant s1split(const void* xy_fn, const char* name) {
 template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D, bool HI = false,
           int RX = 4, int UX = 1, int UY = 1>
 Sweep1Variant s1s(const char* name) {
-    return s1split<P, LIT, TX, TY, NT, TXC, TYC, D>(
-        (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY>, name);
+    Sweep1Variant v = s1split<P, LIT, TX, TY, NT, TXC, TYC, D>(
+        (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1>, name);
+    v.fn_img[0] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 0>;
+    v.fn_img[1] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1>;
+    return v;
 }
 }  // namespace
 
@@ -1504,15 +1513,17 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, false, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
         s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
-        // literal default: tall narrow tiles (x pass over 128 rows for 108 outputs); the
-        // 8-output x items are faster at 512^3 but spill, and lose on larger volumes
+        // literal default: 32x60 tiles, 15 outputs per y item (the xy kernel specialised
+        // for the image dtype: 2.36 ms at 512^3, 15.0 at 1024x1024x768)
+        s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, true>("p21_lit_hi_32x60r15"),
+        s1s<21, true, 32, 60, 256, 10, 2, 256, 1, 3, true>("p21_lit_hi_32x60r10"),
+        s1s<21, true, 32, 60, 256, 30, 2, 256, 1, 3, true>("p21_lit_hi_32x60r30"),
         s1s<21, true, 16, 108, 256, 9, 2, 256, 1, 3, true>("p21_lit_hi_16x108r9"),
         s1s<21, true, 16, 108, 256, 9, 2, 256, 1, 3, true, 8>("p21_lit_hi_16x108x8"),
-        s1s<21, true, 32, 60, 256, 10, 2, 256, 1, 3, true>("p21_lit_hi_32x60r10"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true, 8>("p21_lit_hi_32x44x8"),
         s1s<21, true, 32, 28, 256, 7, 4, 256, 1, 3>("p21_lit_split_32x28w4"),
-        s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, true>("p21_lit_hi_32x60r15"),
+
         s1s<21, true, 64, 28, 512, 7, 4, 256, 1, 3>("p21_lit_split_64x28w4"),
         s1v<21, false, 32, 16, 512, 4>("p21_fast_32x16w4"),
         s1v<21, false, 32, 32, 1024, 4>("p21_fast_32x32t1k"),
@@ -1720,7 +1731,8 @@ int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, cons
             void* a1[] = {(void*)&a.tr, (void*)&a.img, (void*)&a.img_u8, (void*)&xy, (void*)&stride,
                           (void*)&tp.d_blocks.p, (void*)&tp.d_tiles1.p, (void*)&ntiles, (void*)&tp.zc1,
                           (void*)&d.taps};
-            RSF_HIP(hipLaunchKernel(v.fn, dim3(tp.grid1), dim3(v.nt), a1, 0, s));
+            const void* f = v.fn_img[a.img_u8] ? v.fn_img[a.img_u8] : v.fn;
+            RSF_HIP(hipLaunchKernel(f, dim3(tp.grid1), dim3(v.nt), a1, 0, s));
         }
         const int n5 = (int)tp.tiles5.size();
         if (a.out_fits < 0 || a.out_fits > 2) return set_error(RSF_ERR_INVALID_ARGUMENT, "sweep-1 output mode");

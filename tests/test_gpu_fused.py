@@ -234,7 +234,7 @@ def test_kernel_variants_give_identical_results(ctx, dtype, mode, monkeypatch):
     else:
         combos = [(None, None), ("p21_lit_hi_32x44w2:16", None), ("p21_lit_split_32x28w4:64", "lit_split_64x4:16"),
                   ("p21_lit_32x16:64", "lit_64x4:16"), ("p21_lit_hi_32x60r15:8", None),
-                  ("p21_lit_hi_32x60r10:32", None), ("p21_lit_hi_16x108x8:16", None)]
+                  ("p21_lit_hi_16x108r9:32", None), ("p21_lit_hi_16x108x8:16", None), ("p21_lit_hi_32x60r30:16", None)]
     for s1, s2 in combos:
         for var, val in (("RSF_SWEEP1", s1), ("RSF_SWEEP2", s2)):
             if val is None:
