@@ -110,7 +110,7 @@ struct Sweep1Variant {
     const char* name;
     const void* fn2;       // split variants: the z kernel (else null)
     int tx2, ty2;          // z-kernel columns per work item
-    const void* fn2s[6];   // z kernel by 2 * output mode + (image is uint8)
+    const void* fn2s[8];   // z kernel by 2 * output mode + (image is uint8)
     const void* fn_img[2]; // split variants: the xy kernel by (image is uint8), else null
 };
 struct Sweep2Variant {

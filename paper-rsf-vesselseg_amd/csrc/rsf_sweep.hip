@@ -569,8 +569,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 // z kernel: one thread per (x, y) column of a block, TXC x TYC columns per work
 // item, marching all l planes.  Columns inside the valid xy region run the z
 // ring over the xy-blurred channels; every voxel outside the valid region is
-// in the pad band (fits 30/30 = 1).  U8: image dtype; MODE 0 writes F, 1 the
-// fits, 2 the fits and the blurred Heavisides.  All accesses are buffer
+// in the pad band (fits 30/30 = 1).  U8: image dtype; MODE 0 writes F with
+// lambda1 == lambda2, 3 F with separate lambdas, 1 the fits, 2 the fits and
+// the blurred Heavisides (the lambda form compiled in keeps the plane loop one
+// basic block).  All accesses are buffer
 // operations: the column's offset in a VGPR, the plane's offset in an SGPR
 // (blocks stay below 2^32 bytes per buffer).
 template <int P, bool LIT, int TXC, int TYC, int D, bool U8, int MODE>
@@ -583,6 +585,7 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
     constexpr int M = (P - 1) / 2;
     constexpr int ESZ = U8 ? 1 : 4;
     static_assert(P % D == 0, "P must be a multiple of the load queue depth");
+    (void)same_lambda;   // the host picks MODE 0 or 3 from it
     for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
         const Tile tl = tiles[item];
         if (tl.b < 0) continue;
@@ -626,8 +629,8 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
                     fast_fits(bI, d[0], d[1], halfS3, fo, fi, bH, bHc);
                 }
             }
-            if constexpr (MODE == 0) {
-                st(r0, z, force_of(Iv, fo, fi, l1, l2, same_lambda));
+            if constexpr (MODE == 0 || MODE == 3) {
+                st(r0, z, force_of(Iv, fo, fi, l1, l2, MODE == 0));
             } else {
                 st(r0, z, fo);
                 st(r1, z, fi);
@@ -1494,6 +1497,8 @@ Sweep1Variant s1split(const void* xy_fn, const char* name) {
     v.fn2s[3] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 1>;
     v.fn2s[4] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 2>;
     v.fn2s[5] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 2>;
+    v.fn2s[6] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 3>;
+    v.fn2s[7] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 3>;
     return v;
 }
 
@@ -1740,7 +1745,8 @@ int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, cons
                       (void*)&a.out0,   (void*)&a.out1,   (void*)&a.out2,   (void*)&a.out3,
                       (void*)&tp.d_blocks.p, (void*)&tp.d_tiles5.p, (void*)&n5, (void*)&d.taps,
                       (void*)&d.halfS3, (void*)&p->lambda1, (void*)&p->lambda2, (void*)&same};
-        RSF_HIP(hipLaunchKernel(v.fn2s[2 * a.out_fits + a.img_u8], dim3(tp.grid5), dim3(v.tx2 * v.ty2), a2, 0, s));
+        const int zmode = a.out_fits == 0 && !same ? 3 : a.out_fits;
+        RSF_HIP(hipLaunchKernel(v.fn2s[2 * zmode + a.img_u8], dim3(tp.grid5), dim3(v.tx2 * v.ty2), a2, 0, s));
         return RSF_OK;
     }
     void* args[] = {(void*)&a.tr,       (void*)&a.img,     (void*)&a.img_u8, (void*)&a.blurI,

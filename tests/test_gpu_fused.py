@@ -72,7 +72,8 @@ def test_fused_lambda1_ne_lambda2_and_sigma2(ctx):
     import torch
 
     img, phi0, _ = scene((36, 38, 40), tubes=5, seed=3)
-    for kw in (dict(lambda1=1.3, lambda2=0.7), dict(sigma2=1.0), dict(sigma2=1.0, mode=rsf_amd.MODE_LITERAL)):
+    for kw in (dict(lambda1=1.3, lambda2=0.7), dict(lambda1=1.3, lambda2=0.7, mode=rsf_amd.MODE_FAST),
+               dict(sigma2=1.0), dict(sigma2=1.0, mode=rsf_amd.MODE_LITERAL)):
         p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=1, **kw)
         d_phi, d_img = to_dev(phi0), to_dev(img)
         d_out = torch.empty_like(d_phi)
@@ -81,6 +82,9 @@ def test_fused_lambda1_ne_lambda2_and_sigma2(ctx):
         ref = oracle.step(phi0, img.astype(np.float32), oracle_params(p))
         err, tol = one_step_gate(got, ref)
         assert err <= tol, (kw, err, tol)
+        if "lambda2" in kw and p.mode == rsf_amd.MODE_LITERAL:
+            # separate lambdas take their own force form (kernel.cu:44-45), still bit for bit
+            assert np.array_equal(got, ref), (kw, err)
 
 
 @pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
