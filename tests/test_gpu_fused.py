@@ -218,17 +218,20 @@ def test_unaligned_buffers_match_aligned(ctx, mode):
     assert np.abs(got_u - got_a).max() <= 1e-6 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("lambdas", [(1.0, 1.0), (1.3, 0.7)])
 @pytest.mark.parametrize("dtype", [np.uint8, np.float32])
 @pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
-def test_kernel_variants_give_identical_results(ctx, dtype, mode, monkeypatch):
+def test_kernel_variants_give_identical_results(ctx, dtype, mode, lambdas, monkeypatch):
     """The fused sweep 1 and the split xy/z kernels, and every sweep-2 interior
     kernel (vector, scalar buffer, direct), compute the same operations in the
-    same order: a run gives bit-identical phi whichever variants it takes."""
+    same order: a run gives bit-identical phi whichever variants it takes (with
+    equal lambdas and with separate ones, whose force forms differ)."""
     import torch
 
     shape = (40, 44, 48)
     img, phi0, _ = scene(shape, tubes=6, seed=23)
-    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=3, mode=mode)
+    p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=3, mode=mode, lambda1=lambdas[0],
+                            lambda2=lambdas[1])
     d_img = to_dev(img.astype(dtype))
     results = {}
     if mode == rsf_amd.MODE_FAST:
