@@ -1518,6 +1518,7 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, false, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
         s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
+        s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 7>("p21_fast_split_64x44d7"),
         // literal default: 32x60 tiles, 15 outputs per y item (the xy kernel specialised
         // for the image dtype: 2.36 ms at 512^3, 15.0 at 1024x1024x768)
         s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, true>("p21_lit_hi_32x60r15"),

@@ -237,7 +237,7 @@ def test_kernel_variants_give_identical_results(ctx, dtype, mode, lambdas, monke
     if mode == rsf_amd.MODE_FAST:
         combos = [(None, None), ("p21_fast_32x16w4:64", None), (None, "fast_split_64x4:16"),
                   (None, "fast_64x4:16"), ("p21_fast_32x16w4:16", "fast_vec_32x8:8"),
-                  ("p21_fast_split_64x28w4:8", None)]
+                  ("p21_fast_split_64x28w4:8", None), ("p21_fast_split_64x44d7:16", None)]
     else:
         combos = [(None, None), ("p21_lit_hi_32x44w2:16", None), ("p21_lit_split_32x28w4:64", "lit_split_64x4:16"),
                   ("p21_lit_32x16:64", "lit_64x4:16"), ("p21_lit_hi_32x60r15:8", None),
