@@ -284,6 +284,30 @@ def test_config1_full_run_mask_gate(ctx, mode):
         assert dis <= 4e-4, dis
 
 
+@pytest.mark.parametrize("shape,phantom,iterations", [
+    ((512, 512, 512), dict(tubes=300, rmin=2.0, rmax=6.0, bias=20.0, noise=15.0, seed=2), 2),
+    ((768, 1024, 1024), dict(tubes=2000, rmin=1.5, rmax=8.0, bias=20.0, noise=20.0, seed=5), 1),
+], ids=["512cube", "1024x1024x768"])
+def test_benchmark_volumes_literal_bit_identical(ctx, shape, phantom, iterations):
+    """The volumes the benchmark times -- 512^3 (300 tubes, radius 2-6, +-20
+    ramp, noise 15, seed 2) and 1024x1024x768 (2000 tubes, radius 1.5-8, noise
+    20, seed 5), phi0 the signed distance of the thresholded sigma-1 blur --
+    in LITERAL mode with the default kernels for their sizes (the larger one
+    takes the long xy chunks): phi is identical to the oracle's bit for bit
+    (the oracle runs multi-threaded)."""
+    img, _ = rsf_amd.synth_tubes(shape, **phantom)
+    phi0 = rsf_amd.initial_phi(img, 1.0, 110.0)
+    p = rsf_amd.make_params(sigma=3.0, lambda1=1.0, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0,
+                            mode=rsf_amd.MODE_LITERAL, iterations=iterations)
+    print(f"benchmark volume {shape} literal x{iterations}", flush=True)
+    d_img, d_phi = to_dev(img), to_dev(phi0)
+    ctx.run(d_img, d_phi, p)
+    got = d_phi.cpu().numpy()
+    del d_img, d_phi
+    ref = oracle.run(phi0, img.astype(np.float32), oracle_params(p), iterations)
+    assert np.array_equal(got, ref), float(np.abs(got - ref).max())
+
+
 def test_nonfinite_phi_is_reported(ctx):
     """A NaN in phi0 spreads through the blurs; the run reports RSF_ERR_NONFINITE
     instead of returning garbage as a success (single volume and batched)."""
