@@ -308,6 +308,25 @@ def test_benchmark_volumes_literal_bit_identical(ctx, shape, phantom, iterations
     assert np.array_equal(got, ref), float(np.abs(got - ref).max())
 
 
+def test_benchmark_volume_fast_one_step_gate(ctx):
+    """FAST mode on the 512^3 benchmark volume: one step within the gate
+    (max|dphi| <= 1e-4 max|phi|), and within 2e-5 max|phi| of the oracle."""
+    n = 512
+    img, _ = rsf_amd.synth_tubes((n, n, n), tubes=300, rmin=2.0, rmax=6.0, bias=20.0, noise=15.0, seed=2)
+    phi0 = rsf_amd.initial_phi(img, 1.0, 110.0)
+    p = rsf_amd.make_params(sigma=3.0, lambda1=1.0, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0,
+                            mode=rsf_amd.MODE_FAST, iterations=1)
+    print("benchmark volume 512^3 fast x1", flush=True)
+    d_img, d_phi = to_dev(img), to_dev(phi0)
+    ctx.run(d_img, d_phi, p)
+    got = d_phi.cpu().numpy()
+    del d_img, d_phi
+    ref = oracle.run(phi0, img.astype(np.float32), oracle_params(p), 1)
+    err, tol = one_step_gate(got, ref)
+    assert err <= tol, (err, tol)
+    assert err <= 2e-5 * float(np.abs(ref).max()), err
+
+
 def test_nonfinite_phi_is_reported(ctx):
     """A NaN in phi0 spreads through the blurs; the run reports RSF_ERR_NONFINITE
     instead of returning garbage as a success (single volume and batched)."""
