@@ -106,7 +106,12 @@ def plain_run(rsf_amd, torch, ctx, img, phi0, steps, warmup, mode, tune):
                 inside=inside)
 
 
-def batched_run(rsf_amd, torch, ctx, steps, mode, nblocks=64):
+def batched_blocks(rsf_amd, nblocks=64):
+    """The batched workload: nblocks uneven blocks, each side drawn from
+    {96, 160, 192, 256} (numpy default_rng(3)), block b a tube phantom of seed
+    1000+b with tubes in proportion to its volume (300 per 512^3, at least 4),
+    phi0 the signed distance of the thresholded sigma-1 blur.  Returns (shapes
+    (l, h, w), packed blocks, image, phi0) as flat numpy arrays."""
     import numpy as np
 
     rng = np.random.default_rng(3)
@@ -121,6 +126,12 @@ def batched_run(rsf_amd, torch, ctx, steps, mode, nblocks=64):
         bi, _ = rsf_amd.synth_tubes(shp, tubes=tubes, rmin=2.0, rmax=6.0, bias=20.0, noise=15.0, seed=1000 + b)
         img[off:off + nv] = bi.ravel()
         phi[off:off + nv] = rsf_amd.initial_phi(bi, 1.0, 110.0).ravel()
+    return shapes, blocks, img, phi
+
+
+def batched_run(rsf_amd, torch, ctx, steps, mode, nblocks=64):
+    shapes, blocks, img, phi = batched_blocks(rsf_amd, nblocks)
+    total = img.size
     log(f"generated {nblocks} blocks, {total / 1e6:.1f} Mvoxel")
     dev = torch.device("cuda:0")
     d_img = torch.from_numpy(img).to(dev)

@@ -102,12 +102,14 @@ int rsf_context_set_stream(rsf_context* ctx, void* hip_stream);
  * rsf_step_adjoint_device over one image.  No counterpart in the reference. */
 int rsf_context_hold_image(rsf_context* ctx, const void* image);
 int rsf_context_release_image(rsf_context* ctx);
-/* Bytes of device workspace a plain run of this size allocates. */
+/* Bytes of device workspace a plain run (rsf_run_device) of this size
+ * allocates with the default kernels -- an upper bound for tuned kernel
+ * choices -- not counting the tile lists (well under 1 MB). */
 int rsf_workspace_size(const rsf_params* p, int64_t w, int64_t h, int64_t l, int64_t* bytes);
 
 /* ---------------------------------------------------------- fused engine */
 /* One iteration phi_out = step(phi_in) -- CUDA_RSF/main.cpp:234-300.
- * phi_in and phi_out must not alias. */
+ * phi_in and phi_out must not overlap. */
 int rsf_step_device(rsf_context* ctx, const void* image, int image_dtype, const float* phi_in,
                     float* phi_out, int64_t w, int64_t h, int64_t l, const rsf_params* p);
 /* p->iterations steps in place on phi -- the loop of CUDA_RSF/main.cpp:222-314,
@@ -149,6 +151,15 @@ int rsf_profile_sweeps(rsf_context* ctx, const rsf_block* blocks, int nblocks, c
  * choice (nullable) receives a short description of the winner. */
 int rsf_autotune(rsf_context* ctx, const rsf_block* blocks, int nblocks, int image_dtype,
                  const rsf_params* p, char* choice, int choice_len);
+
+/* Kernel variants compiled into the library: the candidates rsf_autotune
+ * times and RSF_SWEEP1 / RSF_SWEEP2 (= "<name>:<z chunk>") force.  sweep is 1
+ * (region statistics) or 2 (stencil update).  rsf_variant_info writes the
+ * name (NUL-terminated, truncated to name_len), the tap count it is built
+ * for (0 for sweep 2: every tap count) and whether it runs the LITERAL mode.
+ * No counterpart in the reference (it has one kernel per operation). */
+int rsf_variant_count(int sweep, int* count);
+int rsf_variant_info(int sweep, int index, char* name, int name_len, int* taps, int* literal);
 
 /* ------------------------------------------------------- block pipeline
  * The gigavoxel workflow of README.md:6-14,56-59: cut a volume into
@@ -232,7 +243,8 @@ int rsf_step_adjoint_device(rsf_context* ctx, const rsf_block* blocks, int nbloc
                             const void* image, int image_dtype, const float* phi,
                             const float* grad_out, float* grad_in, float* grad_params,
                             const rsf_params* p);
-/* Forward step over blocks (for checkpointed recomputation). */
+/* Forward step over blocks (for checkpointed recomputation).  phi_in and
+ * phi_out (each spanning the packed blocks) must not overlap. */
 int rsf_step_blocks_device(rsf_context* ctx, const rsf_block* blocks, int nblocks,
                            const void* image, int image_dtype, const float* phi_in,
                            float* phi_out, const rsf_params* p);

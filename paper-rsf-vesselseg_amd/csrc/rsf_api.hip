@@ -318,6 +318,12 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
         *s2 = find_variant2(literal ? "lit_split_64x4" : "fast_split_64x4");
 }
 
+// true when the float ranges [a, a+n) and [b, b+n) share an element
+bool ranges_overlap(const float* a, const float* b, int64_t n) {
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b, bytes = (uintptr_t)n * sizeof(float);
+    return x < y + bytes && y < x + bytes;
+}
+
 // true when every pointer is 16-byte aligned (null pointers are ignored)
 bool aligned16(std::initializer_list<const void*> ptrs) {
     for (const void* q : ptrs)
@@ -419,12 +425,18 @@ int ensure_workspace(rsf_context* c, const Prepared& pr) {
     return RSF_OK;
 }
 
+// The float copy and image blur of the workspace are about to be overwritten:
+// whatever preparation the held image had there is gone.
+void invalidate_held(rsf_context* c) { c->held_key.clear(); }
+
 // Once per run: image as float and, in fast mode, the padded blur of the image.
 int prepare_image(rsf_context* c, const Prepared& pr, const void* image, int u8) {
     if (pr.literal && pr.sw == 0) return RSF_OK;
     // a held image keeps its preparation while the layout, taps, mode and workspace match
     std::string key;
-    if (image == c->held_image) {
+    if (image != c->held_image) {
+        invalidate_held(c);
+    } else {
         std::ostringstream os;
         os << blocks_key(pr.blocks) << '|' << u8 << '|' << pr.literal << '|' << pr.sw << '|' << c->ws_epoch << '|';
         for (int k = 0; k < pr.d.P; ++k) {
@@ -546,10 +558,16 @@ int run_prepared(rsf_context* c, const Prepared& pr, const void* image, int u8, 
     const int T = p->iterations;
     if (T == 0) return RSF_OK;
     Runner* r = pr.runner;
+    // The captured kernels hold the parameters by value: the key carries their
+    // exact bit patterns (values that print alike must not share a graph).
     std::ostringstream gk;
-    gk << image << '|' << u8 << '|' << (const void*)phi << '|' << p->sigma << '|' << p->lambda1 << '|'
-       << p->lambda2 << '|' << p->nu << '|' << p->mu << '|' << p->dt << '|' << p->epsilon << '|' << p->sigma2
-       << '|' << T << '|' << (iter_ms ? 1 : 0);
+    gk << image << '|' << u8 << '|' << (const void*)phi << '|' << std::hex;
+    for (float v : {p->sigma, p->lambda1, p->lambda2, p->nu, p->mu, p->dt, p->epsilon, p->sigma2}) {
+        uint32_t bits;
+        std::memcpy(&bits, &v, 4);
+        gk << bits << '|';
+    }
+    gk << std::dec << p->mode << '|' << T << '|' << (iter_ms ? 1 : 0);
     if (r->graph_key != gk.str() || r->ws_epoch != c->ws_epoch) {
         r->graphs.reset();
         r->graph_key.clear();
@@ -731,12 +749,47 @@ int rsf_workspace_size(const rsf_params* p, int64_t w, int64_t h, int64_t l, int
     if (!bytes) return set_error(RSF_ERR_INVALID_ARGUMENT, "bytes is NULL");
     Derived d;
     RSF_TRY(derive(p, &d));
+    if (w < d.P || h < d.P || l < d.P) return set_error(RSF_ERR_INVALID_ARGUMENT, "dims smaller than the blur");
     const int64_t n = w * h * l;
+    const int literal = p->mode == RSF_MODE_LITERAL;
+    const int s1 = default_sweep1(d.P, literal);
+    if (s1 < 0) {
+        // no fused kernel for this tap count: the unfused pipeline's image copy,
+        // phi ping-pong and 20 intermediate volumes
+        *bytes = 22 * n * (int64_t)sizeof(float);
+        return RSF_OK;
+    }
     int64_t vols = 3;  // phi ping-pong, transform, force
-    if (p->mode == RSF_MODE_FAST) vols += 2;           // blur(I), image as float
-    if ((int)p->sigma2 > 0) vols += 4 + (p->mode == RSF_MODE_FAST ? 0 : 1);
-    int64_t tmp = (p->mode == RSF_MODE_FAST) ? 2 * (w - d.P + 1) * h * l : 0;
+    if (!literal) vols += 2;                       // blur(I), image as float
+    if ((int)p->sigma2 > 0) vols += 4 + (literal ? 1 : 0);   // fo, fi, Eo, Ei (+ image as float)
+    // the split sweep 1 holds its xy-blurred channels (4 literal, 2 fast); blocks of
+    // 2^30 voxels and more run the fused kernel, which has none (pick_variants)
+    const bool small = 4.0 * ((double)n + 4.0 * w * h + 4.0 * w + 64.0) < 4294967295.0;
+    bool split = false;
+    for (const auto& v : sweep1_variants()) split = split || (v.fn2 && v.P == d.P && v.literal == literal);
+    if (small && split) vols += literal ? 4 : 2;
+    // fast mode: the two pass temporaries of the image blur
+    const int64_t tmp = literal ? 0 : 2 * (w - d.P + 1) * h * l;
     *bytes = (vols * n + tmp) * (int64_t)sizeof(float);
+    return RSF_OK;
+}
+
+int rsf_variant_count(int sweep, int* count) {
+    if (!count) return set_error(RSF_ERR_INVALID_ARGUMENT, "count is NULL");
+    if (sweep == 1) *count = (int)sweep1_variants().size();
+    else if (sweep == 2) *count = (int)sweep2_variants().size();
+    else return set_error(RSF_ERR_INVALID_ARGUMENT, "sweep must be 1 or 2, got %d", sweep);
+    return RSF_OK;
+}
+
+int rsf_variant_info(int sweep, int index, char* name, int name_len, int* taps, int* literal) {
+    int n = 0;
+    RSF_TRY(rsf_variant_count(sweep, &n));
+    if (index < 0 || index >= n) return set_error(RSF_ERR_INVALID_ARGUMENT, "variant %d of %d", index, n);
+    const char* nm = sweep == 1 ? sweep1_variants()[index].name : sweep2_variants()[index].name;
+    if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", nm);
+    if (taps) *taps = sweep == 1 ? sweep1_variants()[index].P : 0;
+    if (literal) *literal = sweep == 1 ? sweep1_variants()[index].literal : sweep2_variants()[index].literal;
     return RSF_OK;
 }
 
@@ -763,11 +816,15 @@ int rsf_step_blocks_device(rsf_context* c, const rsf_block* blocks, int nblocks,
     if (!image || !phi_in || !phi_out) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
     Prepared pr;
     RSF_TRY(prepare(c, blocks, nblocks, p, aligned16({phi_in, phi_out}), &pr));
+    if (ranges_overlap(phi_in, phi_out, pr.extent))
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "phi_in and phi_out overlap: sweep 2 reads neighbours of "
+                                                   "voxels other workgroups are writing");
     const int u8 = image_dtype == RSF_DTYPE_U8;
     if (!pr.fused) {
         if (nblocks != 1) return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel", pr.d.P);
         const BlockDesc& b = pr.blocks[0];
         RSF_TRY(ensure_ws(c, c->imgf, (size_t)pr.extent * sizeof(float)));
+        invalidate_held(c);
         RSF_TRY(ops::to_float(c->imgf.as<float>(), image, u8, pr.extent, c->stream));
         RSF_TRY(unfused_step(c, c->imgf.as<float>() + b.off, phi_in + b.off, phi_out + b.off, b.w, b.h, b.l, p, pr.d));
         RSF_HIP(hipStreamSynchronize(c->stream));
@@ -794,6 +851,7 @@ int rsf_run_device(rsf_context* c, const void* image, int image_dtype, float* ph
         // unfused pipeline for tap counts without a fused specialisation
         RSF_TRY(ensure_ws(c, c->imgf, (size_t)pr.extent * sizeof(float)));
         RSF_TRY(ensure_ws(c, c->phi2, (size_t)pr.extent * sizeof(float)));
+        invalidate_held(c);
         RSF_TRY(ops::to_float(c->imgf.as<float>(), image, u8, pr.extent, c->stream));
         float* A = phi;
         float* B = c->phi2.as<float>();
@@ -1022,9 +1080,10 @@ int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks
     RSF_TRY(check_dtype(image_dtype));
     if (!image || !phi || !grad_out || !grad_in || !grad_params)
         return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
-    if (grad_in == grad_out || grad_in == phi) return set_error(RSF_ERR_INVALID_ARGUMENT, "grad_in must not alias");
     Prepared pr;
     RSF_TRY(prepare(c, blocks, nblocks, p, aligned16({phi}), &pr));
+    if (ranges_overlap(grad_in, grad_out, pr.extent) || ranges_overlap(grad_in, phi, pr.extent))
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "grad_in must not overlap grad_out or phi");
     if (!pr.fused) return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel", pr.d.P);
     if (pr.sw != 0) return set_error(RSF_ERR_UNSUPPORTED, "the device adjoint covers sigma2 = 0");
     const int u8 = image_dtype == RSF_DTYPE_U8;

@@ -606,15 +606,18 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         const __amdgpu_buffer_rsrc_t r2 = rsrc(out2 + bd.off), r3 = rsrc(out3 + bd.off);
         const int c = y * w + x;                 // element offset of the column
         const int vc = 4 * c, vci = ESZ * c;     // byte offsets (VGPR)
+        // plane byte offsets (SGPR) in unsigned 32-bit arithmetic: a block spans
+        // fewer than 2^32 bytes per buffer, but 4 * z * plane can exceed 2^31
+        auto so4 = [&](int z) { return (int)(4u * (uint32_t)z * (uint32_t)plane); };
         auto ldf = [&](__amdgpu_buffer_rsrc_t r, int z) {
-            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vc, 4 * z * plane, 0));
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vc, so4(z), 0));
         };
         auto ldi = [&](int z) {
-            if constexpr (U8) return (float)__builtin_amdgcn_raw_buffer_load_b8(rI, vci, z * plane, 0);
-            else return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rI, vci, 4 * z * plane, 0));
+            if constexpr (U8) return (float)__builtin_amdgcn_raw_buffer_load_b8(rI, vci, (int)((uint32_t)z * (uint32_t)plane), 0);
+            else return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rI, vci, so4(z), 0));
         };
         auto st = [&](__amdgpu_buffer_rsrc_t r, int z, float v) {
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, vc, 4 * z * plane, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, vc, so4(z), 0);
         };
         // ring outputs d[ch] (valid) or the pad band; bI: image blur (fast mode)
         auto store_out = [&](int z, float Iv, bool valid, float bI, const float (&d)[CH]) {

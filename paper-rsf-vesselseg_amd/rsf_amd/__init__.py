@@ -101,6 +101,8 @@ _SIGS = {
     "rsf_run_blocks_sequential_device": (_I, [_P, _PB, _I, _P, _I, _P, _PP]),
     "rsf_autotune": (_I, [_P, _PB, _I, _I, _PP, C.c_char_p, _I]),
     "rsf_profile_sweeps": (_I, [_P, _PB, _I, _P, _I, _P, _PP, C.POINTER(_F), C.POINTER(_F)]),
+    "rsf_variant_count": (_I, [_I, C.POINTER(_I)]),
+    "rsf_variant_info": (_I, [_I, _I, C.c_char_p, _I, C.POINTER(_I), C.POINTER(_I)]),
     "rsf_step_adjoint_device": (_I, [_P, _PB, _I, _P, _I, _P, _P, _P, _P, _PP]),
     "rsf_step_blocks_device": (_I, [_P, _PB, _I, _P, _I, _P, _P, _PP]),
     "rsf_block_grid": (_I, [_I64] * 8 + [_PB, _P, _I, C.POINTER(_I), C.POINTER(_I64)]),
@@ -185,6 +187,35 @@ def _sync(*arrays):
             return
 
 
+def _device_tensor(t, name, numel=None, dtypes=("torch.float32",), shape=None):
+    """The device-pointer entry points hand their tensors' addresses to HIP
+    kernels: refuse anything that is not a contiguous device tensor of the
+    right dtype holding at least `numel` elements (or exactly `shape`)."""
+    if not (hasattr(t, "is_cuda") and hasattr(t, "data_ptr")):
+        raise TypeError(f"{name}: expected a torch tensor on the HIP device, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: tensor is on {t.device}, not on the HIP device")
+    if str(t.dtype) not in dtypes:
+        raise TypeError(f"{name}: dtype {t.dtype}, expected {' or '.join(dtypes)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if numel is not None and t.numel() < numel:
+        raise ValueError(f"{name}: {t.numel()} elements, the blocks span {numel}")
+    return t
+
+
+_IMAGE_DTYPES = ("torch.uint8", "torch.float32")
+
+
+def _extent(blocks) -> int:
+    """Voxels spanned by packed blocks [(offset, w, h, l), ...]."""
+    if not blocks:
+        raise ValueError("no blocks")
+    return max(int(off) + int(w) * int(h) * int(l) for (off, w, h, l) in blocks)
+
+
 def _dtype_code(a) -> int:
     s = str(getattr(a, "dtype", ""))
     if s in ("uint8", "torch.uint8"):
@@ -192,6 +223,20 @@ def _dtype_code(a) -> int:
     if s in ("float32", "torch.float32"):
         return DTYPE_F32
     raise TypeError(f"image dtype must be uint8 or float32, got {s}")
+
+
+def variants(sweep: int):
+    """Kernel variants of sweep 1 or 2 compiled into the library: a list of
+    (name, taps, literal) -- taps 0 for sweep 2 (any tap count)."""
+    n = C.c_int()
+    check(lib().rsf_variant_count(int(sweep), C.byref(n)))
+    out = []
+    for k in range(n.value):
+        buf = C.create_string_buffer(64)
+        P, lit = C.c_int(), C.c_int()
+        check(lib().rsf_variant_info(int(sweep), k, buf, 64, C.byref(P), C.byref(lit)))
+        out.append((buf.value.decode(), P.value, bool(lit.value)))
+    return out
 
 
 def taps(sigma: float):
@@ -268,13 +313,22 @@ class Context:
 
     # ---- device-pointer forms (torch tensors on the HIP device)
     def step(self, image, phi_in, phi_out, params: Params):
+        _device_tensor(phi_in, "phi_in")
+        if phi_in.dim() != 3:
+            raise ValueError("phi_in must be a (l, h, w) volume")
         l, h, w = phi_in.shape
+        _device_tensor(phi_out, "phi_out", shape=phi_in.shape)
+        _device_tensor(image, "image", dtypes=_IMAGE_DTYPES, shape=phi_in.shape)
         _sync(image, phi_in, phi_out)
         check(lib().rsf_step_device(self._h, _ptr(image), _dtype_code(image), _ptr(phi_in), _ptr(phi_out),
                                     w, h, l, C.byref(params)))
 
     def run(self, image, phi, params: Params):
+        _device_tensor(phi, "phi")
+        if phi.dim() != 3:
+            raise ValueError("phi must be a (l, h, w) volume")
         l, h, w = phi.shape
+        _device_tensor(image, "image", dtypes=_IMAGE_DTYPES, shape=phi.shape)
         _sync(image, phi)
         check(lib().rsf_run_device(self._h, _ptr(image), _dtype_code(image), _ptr(phi), w, h, l,
                                    C.byref(params), None))
@@ -287,6 +341,9 @@ class Context:
 
     def run_batched(self, blocks, image, phi, params: Params):
         arr = blocks_array(blocks)
+        n = _extent(blocks)
+        _device_tensor(phi, "phi", numel=n)
+        _device_tensor(image, "image", numel=n, dtypes=_IMAGE_DTYPES)
         _sync(image, phi)
         check(lib().rsf_run_batched_device(self._h, arr, len(blocks), _ptr(image), _dtype_code(image),
                                            _ptr(phi), C.byref(params)))
@@ -294,6 +351,9 @@ class Context:
 
     def run_sequential(self, blocks, image, phi, params: Params):
         arr = blocks_array(blocks)
+        n = _extent(blocks)
+        _device_tensor(phi, "phi", numel=n)
+        _device_tensor(image, "image", numel=n, dtypes=_IMAGE_DTYPES)
         _sync(image, phi)
         check(lib().rsf_run_blocks_sequential_device(self._h, arr, len(blocks), _ptr(image),
                                                      _dtype_code(image), _ptr(phi), C.byref(params)))
@@ -301,12 +361,21 @@ class Context:
 
     def step_blocks(self, blocks, image, phi_in, phi_out, params: Params):
         arr = blocks_array(blocks)
+        n = _extent(blocks)
+        _device_tensor(phi_in, "phi_in", numel=n)
+        _device_tensor(phi_out, "phi_out", numel=n)
+        _device_tensor(image, "image", numel=n, dtypes=_IMAGE_DTYPES)
         _sync(image, phi_in, phi_out)
         check(lib().rsf_step_blocks_device(self._h, arr, len(blocks), _ptr(image), _dtype_code(image),
                                            _ptr(phi_in), _ptr(phi_out), C.byref(params)))
 
     def adjoint(self, blocks, image, phi, grad_out, grad_in, grad_params, params: Params):
         arr = blocks_array(blocks)
+        n = _extent(blocks)
+        for t, name in ((phi, "phi"), (grad_out, "grad_out"), (grad_in, "grad_in")):
+            _device_tensor(t, name, numel=n)
+        _device_tensor(grad_params, "grad_params", numel=4)
+        _device_tensor(image, "image", numel=n, dtypes=_IMAGE_DTYPES)
         _sync(image, phi, grad_out, grad_in, grad_params)
         check(lib().rsf_step_adjoint_device(self._h, arr, len(blocks), _ptr(image), _dtype_code(image),
                                             _ptr(phi), _ptr(grad_out), _ptr(grad_in), _ptr(grad_params),
@@ -317,6 +386,9 @@ class Context:
         eager iterations (HIP events on the context stream)."""
         arr = blocks_array(blocks)
         a, b = C.c_float(), C.c_float()
+        n = _extent(blocks)
+        _device_tensor(phi, "phi", numel=n)
+        _device_tensor(image, "image", numel=n, dtypes=_IMAGE_DTYPES)
         _sync(image, phi)
         check(lib().rsf_profile_sweeps(self._h, arr, len(blocks), _ptr(image), _dtype_code(image), _ptr(phi),
                                        C.byref(params), C.byref(a), C.byref(b)))
@@ -330,20 +402,30 @@ class Context:
 
     def disassemble(self, volume, grid, packed):
         """Copy the blocks of `grid` (a BlockGrid) out of volume (l, h, w) into packed."""
+        _device_tensor(volume, "volume", dtypes=_IMAGE_DTYPES)
+        if volume.dim() != 3:
+            raise ValueError("volume must be (l, h, w)")
         l, h, w = volume.shape
+        _device_tensor(packed, "packed", numel=_extent(grid.blocks), dtypes=(str(volume.dtype),))
         _sync(volume, packed)
         check(lib().rsf_disassemble_device(self._h, _ptr(volume), _dtype_code(volume), w, h, l, grid.c_blocks,
                                            grid.c_origins.ctypes.data, len(grid.blocks), _ptr(packed)))
 
     def reassemble(self, packed, grid, volume):
         """Blend the packed blocks of `grid` into volume (l, h, w), float32."""
+        _device_tensor(volume, "volume", shape=grid.shape)
         l, h, w = volume.shape
+        _device_tensor(packed, "packed", numel=_extent(grid.blocks))
         _sync(packed, volume)
         check(lib().rsf_reassemble_device(self._h, _ptr(packed), w, h, l, *grid.block_size, grid.overlap,
                                           grid.min_size, _ptr(volume)))
 
     def run_volume_blocks(self, image, phi, block_size, overlap, params: Params):
         """Whole block pipeline in place on phi (l, h, w): block_size = (bw, bh, bl)."""
+        _device_tensor(phi, "phi")
+        if phi.dim() != 3:
+            raise ValueError("phi must be a (l, h, w) volume")
+        _device_tensor(image, "image", dtypes=_IMAGE_DTYPES, shape=phi.shape)
         l, h, w = phi.shape
         bw, bh, bl = block_size
         _sync(image, phi)
@@ -352,6 +434,10 @@ class Context:
 
     def sdf_from_seed(self, seed, phi, shift=13.0):
         """phi0 = sdf(seed - shift) on the device (seed uint8 or float32, phi float32)."""
+        _device_tensor(seed, "seed", dtypes=_IMAGE_DTYPES)
+        if seed.dim() != 3:
+            raise ValueError("seed must be a (l, h, w) volume")
+        _device_tensor(phi, "phi", shape=seed.shape)
         l, h, w = seed.shape
         _sync(seed, phi)
         check(lib().rsf_sdf_from_seed_device(self._h, _ptr(seed), _dtype_code(seed), _ptr(phi), w, h, l, shift))
@@ -371,18 +457,25 @@ class Context:
         uint8 device tensors of that shape receive the vote."""
         z0, y0, x0 = (np.ascontiguousarray(a, np.int64) for a in starts)
         l, h, w = shape
+        _device_tensor(pred, "pred", numel=len(z0) * len(y0) * len(x0) * int(np.prod(tile)))
+        _device_tensor(prob, "prob", dtypes=("torch.float64",), shape=(l, h, w))
+        _device_tensor(mask, "mask", dtypes=("torch.uint8",), shape=(l, h, w))
         _sync(pred, prob, mask)
         check(lib().rsf_tile_vote_device(self._h, _ptr(pred), *tile, z0.ctypes.data, len(z0), y0.ctypes.data,
                                          len(y0), x0.ctypes.data, len(x0), w, h, l, _ptr(prob), _ptr(mask)))
 
     def npy_load(self, path, dst):
         """Stream a 3-D .npy file into the device tensor dst (uint8 as is, else float32)."""
+        _device_tensor(dst, "dst", dtypes=_IMAGE_DTYPES)
         _sync(dst)
         check(lib().rsf_npy_load_device(self._h, os.fsencode(path), _ptr(dst), _dtype_code(dst),
                                         dst.numel() * dst.element_size()))
 
     def npy_save(self, path, src):
         """Stream the float32 device volume src (l, h, w) into a .npy file."""
+        _device_tensor(src, "src")
+        if src.dim() != 3:
+            raise ValueError("src must be a (l, h, w) volume")
         l, h, w = src.shape
         _sync(src)
         check(lib().rsf_npy_save_device(self._h, os.fsencode(path), _ptr(src), w, h, l))
@@ -395,7 +488,12 @@ class Context:
         return ms
 
     def step_unfused(self, image_f32, phi_in, phi_out, params: Params):
+        _device_tensor(phi_in, "phi_in")
+        if phi_in.dim() != 3:
+            raise ValueError("phi_in must be a (l, h, w) volume")
         l, h, w = phi_in.shape
+        _device_tensor(phi_out, "phi_out", shape=phi_in.shape)
+        _device_tensor(image_f32, "image", shape=phi_in.shape)
         _sync(image_f32, phi_in, phi_out)
         check(lib().rsf_step_unfused_device(self._h, _ptr(image_f32), _ptr(phi_in), _ptr(phi_out), w, h, l,
                                             C.byref(params)))

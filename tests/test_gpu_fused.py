@@ -157,11 +157,16 @@ def test_batched_equals_sequential_blocks(ctx, mode):
     torch.cuda.synchronize()
     A, B = a.cpu().numpy(), b.cpu().numpy()
     assert np.array_equal(A, B)
-    # and each block equals the oracle on that block alone
-    for (off, w, h, l), shp in list(zip(blocks, shapes))[:3]:
+    # and each block equals the oracle on that block alone: bit for bit in
+    # literal mode, within the mask gate in fast mode
+    for (off, w, h, l), shp in zip(blocks, shapes):
         ref = oracle.run(phi[off:off + w * h * l].reshape(shp), img[off:off + w * h * l].reshape(shp)
                          .astype(np.float32), oracle_params(p), 6)
-        assert mask_disagreement(A[off:off + w * h * l].reshape(shp), ref) <= 1e-4
+        got = A[off:off + w * h * l].reshape(shp)
+        if mode == rsf_amd.MODE_LITERAL:
+            assert np.array_equal(got, ref), (shp, float(np.abs(got - ref).max()))
+        else:
+            assert mask_disagreement(got, ref) <= 1e-4
 
 
 def test_unfused_device_step_matches_oracle(ctx):
@@ -218,32 +223,41 @@ def test_unaligned_buffers_match_aligned(ctx, mode):
     assert np.abs(got_u - got_a).max() <= 1e-6 * np.abs(ref).max()
 
 
+def _registered(mode, P=21):
+    """Every kernel variant the library can run (and rsf_autotune can pick) for
+    this tap count and mode: sweep-1 names, sweep-2 names."""
+    lit = mode == rsf_amd.MODE_LITERAL
+    s1 = [n for n, taps, l in rsf_amd.variants(1) if taps == P and l == lit]
+    s2 = [n for n, _, l in rsf_amd.variants(2) if l == lit]
+    return s1, s2
+
+
+@pytest.mark.parametrize("shape", [(40, 44, 48), (52, 97, 130)], ids=["40x44x48", "52x97x130"])
 @pytest.mark.parametrize("lambdas", [(1.0, 1.0), (1.3, 0.7)])
 @pytest.mark.parametrize("dtype", [np.uint8, np.float32])
 @pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
-def test_kernel_variants_give_identical_results(ctx, dtype, mode, lambdas, monkeypatch):
-    """The fused sweep 1 and the split xy/z kernels, and every sweep-2 interior
-    kernel (vector, scalar buffer, direct), compute the same operations in the
-    same order: a run gives bit-identical phi whichever variants it takes (with
-    equal lambdas and with separate ones, whose force forms differ)."""
+def test_kernel_variants_give_identical_results(ctx, dtype, mode, lambdas, shape, monkeypatch):
+    """Every registered variant -- each sweep-1 kernel for P = 21 (fused, split
+    xy/z) and each sweep-2 kernel (vector, scalar buffer, direct, LDS), the set
+    rsf_autotune chooses from -- computes the same operations in the same order:
+    a run gives bit-identical phi whichever variants and z chunks it takes, with
+    equal lambdas and with separate ones (whose force forms differ).  The second
+    shape has full and partial tiles and rows that are not 16-byte aligned (w =
+    130), the first aligned rows (the vector sweep-2 interior).  In literal mode
+    the default run is also the oracle's phi bit for bit."""
     import torch
 
-    shape = (40, 44, 48)
     img, phi0, _ = scene(shape, tubes=6, seed=23)
     p = rsf_amd.make_params(sigma=3.0, epsilon=1.0, iterations=3, mode=mode, lambda1=lambdas[0],
                             lambda2=lambdas[1])
     d_img = to_dev(img.astype(dtype))
+    s1, s2 = _registered(mode)
+    combos = [(None, None)]
+    combos += [(f"{n}:{zc}", None) for n in s1 for zc in (8, 32)]
+    combos += [(None, f"{n}:{zc}") for n in s2 for zc in (8, 64)]
     results = {}
-    if mode == rsf_amd.MODE_FAST:
-        combos = [(None, None), ("p21_fast_32x16w4:64", None), (None, "fast_split_64x4:16"),
-                  (None, "fast_64x4:16"), ("p21_fast_32x16w4:16", "fast_vec_32x8:8"),
-                  ("p21_fast_split_64x28w4:8", None), ("p21_fast_split_64x44d7:16", None)]
-    else:
-        combos = [(None, None), ("p21_lit_hi_32x44w2:16", None), ("p21_lit_split_32x28w4:64", "lit_split_64x4:16"),
-                  ("p21_lit_32x16:64", "lit_64x4:16"), ("p21_lit_hi_32x60r15:8", None),
-                  ("p21_lit_hi_16x108r9:32", None), ("p21_lit_hi_16x108x8:16", None), ("p21_lit_hi_32x60r30:16", None)]
-    for s1, s2 in combos:
-        for var, val in (("RSF_SWEEP1", s1), ("RSF_SWEEP2", s2)):
+    for c1, c2 in combos:
+        for var, val in (("RSF_SWEEP1", c1), ("RSF_SWEEP2", c2)):
             if val is None:
                 monkeypatch.delenv(var, raising=False)
             else:
@@ -251,10 +265,54 @@ def test_kernel_variants_give_identical_results(ctx, dtype, mode, lambdas, monke
         d_phi = to_dev(phi0)
         ctx.run(d_img, d_phi, p)
         torch.cuda.synchronize()
-        results[(s1, s2)] = d_phi.cpu().numpy()
+        results[(c1, c2)] = d_phi.cpu().numpy()
     ref = results[(None, None)]
     for k, v in results.items():
         assert np.array_equal(v, ref), (k, float(np.abs(v - ref).max()))
+    if mode == rsf_amd.MODE_LITERAL:
+        want = oracle.run(phi0, img.astype(np.float32), oracle_params(p), 3)
+        assert np.array_equal(ref, want), float(np.abs(ref - want).max())
+
+
+@pytest.mark.parametrize("P", list(range(7, 37, 2)))
+@pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
+def test_every_odd_tap_count_one_step(ctx, mode, P):
+    """sigma = (P + 1/2)/7 gives floor(7 sigma) = P (main.cpp:152-155): every odd
+    tap count from 7 to 35 runs its fused sweep-1 kernel (P = 21 the split
+    default).  LITERAL: one step equals the oracle bit for bit; FAST: within
+    2e-5 max|phi|.  The batched launch of two uneven blocks equals the blocks
+    run one by one."""
+    import torch
+
+    sigma = float(np.float32((P + 0.5) / 7.0))
+    assert rsf_amd.taps(sigma)[1] == P
+    shape = (40, 41, 44)
+    img, phi0, _ = scene(shape, tubes=6, seed=200 + P)
+    p = rsf_amd.make_params(sigma=sigma, epsilon=1.0, mode=mode, iterations=1)
+    d_out = torch.empty(shape, dtype=torch.float32, device="cuda")
+    ctx.step(to_dev(img), to_dev(phi0), d_out, p)
+    torch.cuda.synchronize()
+    got = d_out.cpu().numpy()
+    ref = oracle.step(phi0, img.astype(np.float32), oracle_params(p))
+    if mode == rsf_amd.MODE_LITERAL:
+        assert np.array_equal(got, ref), float(np.abs(got - ref).max())
+    else:
+        err, _ = one_step_gate(got, ref)
+        assert err <= 2e-5 * float(np.abs(ref).max()), err
+    shapes = [shape, (P + 3, P + 8, P + 5)]
+    blocks, total = rsf_amd.pack_blocks(shapes)
+    im = np.empty(total, np.uint8)
+    ph = np.empty(total, np.float32)
+    for k, ((off, w, h, l), shp) in enumerate(zip(blocks, shapes)):
+        bi, bp, _ = scene(shp, tubes=3, seed=300 + P + k)
+        im[off:off + bi.size] = bi.ravel()
+        ph[off:off + bp.size] = bp.ravel()
+    p2 = rsf_amd.make_params(sigma=sigma, epsilon=1.0, mode=mode, iterations=2)
+    a, b = to_dev(ph), to_dev(ph)
+    ctx.run_batched(blocks, to_dev(im), a, p2)
+    ctx.run_sequential(blocks, to_dev(im), b, p2)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
 
 
 @pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
@@ -306,6 +364,51 @@ def test_benchmark_volumes_literal_bit_identical(ctx, shape, phantom, iterations
     del d_img, d_phi
     ref = oracle.run(phi0, img.astype(np.float32), oracle_params(p), iterations)
     assert np.array_equal(got, ref), float(np.abs(got - ref).max())
+
+
+@pytest.mark.parametrize("mode", [rsf_amd.MODE_LITERAL, rsf_amd.MODE_FAST])
+def test_volume_of_2_pow_30_voxels_matches_oracle_on_crops(ctx, mode):
+    """1024^3 voxels (2^30: 4 GiB of phi) takes the kernels with 64-bit
+    addressing -- the fused sweep 1 and the direct sweep 2 -- instead of the
+    32-bit buffer-offset ones.  Two iterations are compared with the oracle on
+    interior crops: a voxel's new phi depends only on phi and the image within
+    12 voxels per iteration (10 for the blur, 2 for the curvature stencil), so at
+    voxels 24 or more from the faces of a crop that itself lies 24 or more inside
+    the volume (away from the pad band and the x-wrap), the oracle run on the
+    crop alone computes exactly the whole-volume result.  LITERAL: bit for bit;
+    FAST: the one-step-gate tolerance.  The crops include the far corner, beyond
+    2^31 elements."""
+    import torch
+
+    n, T, c, mg = 1024, 2, 112, 24
+    print(f"2^30 voxels mode={mode}", flush=True)
+    ar = torch.arange(n, device="cuda", dtype=torch.float32)
+    z, y, x = ar.view(n, 1, 1), ar.view(1, n, 1), ar.view(1, 1, n)
+    f = torch.sin(x / 13.0) * torch.sin(y / 17.0 + 0.5) * torch.sin(z / 11.0 + 1.0)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(30)
+    noise = torch.randn((n, n, n), device="cuda", generator=gen) * 15.0
+    d_img = torch.clamp(torch.round(30.0 + 170.0 * (f > 0.3).float() + x * 0.02 + noise), 0, 255).to(torch.uint8)
+    del noise
+    d_phi = (8.0 * (0.3 - f)).contiguous()
+    del f
+    starts = [(mg, mg, mg), (n // 2 - c // 2, n // 2 - 40, n // 2 + 17), (n - mg - c, n - mg - c, n - mg - c)]
+    crops = [(d_img[a:a + c, b:b + c, e:e + c].cpu().numpy(), d_phi[a:a + c, b:b + c, e:e + c].cpu().numpy())
+             for a, b, e in starts]
+    p = rsf_amd.make_params(sigma=3.0, lambda1=1.0, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0, mode=mode,
+                            iterations=T)
+    ctx.run(d_img, d_phi, p)
+    torch.cuda.synchronize()
+    got = [d_phi[a + mg:a + c - mg, b + mg:b + c - mg, e + mg:e + c - mg].cpu().numpy() for a, b, e in starts]
+    del d_img, d_phi
+    torch.cuda.empty_cache()
+    for (ci, cp), g in zip(crops, got):
+        ref = oracle.run(cp, ci.astype(np.float32), oracle_params(p), T)[mg:c - mg, mg:c - mg, mg:c - mg]
+        if mode == rsf_amd.MODE_LITERAL:
+            assert np.array_equal(g, ref), float(np.abs(g - ref).max())
+        else:
+            err, tol = one_step_gate(g, ref)
+            assert err <= tol, (err, tol)
 
 
 def test_benchmark_volume_fast_one_step_gate(ctx):

@@ -219,3 +219,89 @@ def test_npy_info_reads_headers(tmp_path):
     (tmp_path / "bad.npy").write_bytes(b"nope")
     with pytest.raises(rsf_amd.RSFError, match="not a .npy"):
         rsf_amd.npy_info(str(tmp_path / "bad.npy"))
+
+
+def test_variant_query_names_the_tuner_candidates():
+    """The variant query lists what rsf_autotune times: split and fused sweep-1
+    kernels for P = 21 in both modes (among them the tuner's batched and
+    large-volume choices), a fused sweep-1 kernel for every odd tap count up to
+    35, and sweep-2 kernels of both modes; out-of-range queries are errors."""
+    v1, v2 = rsf_amd.variants(1), rsf_amd.variants(2)
+    names1 = {n for n, _, _ in v1}
+    assert {"p21_lit_hi_32x44x8", "p21_lit_hi_32x60r15", "p21_fast_split_64x44w4"} <= names1
+    for P in range(7, 37, 2):
+        for lit in (False, True):
+            assert any(t == P and l == lit for _, t, l in v1), (P, lit)
+    assert all(t == 0 for _, t, _ in v2)
+    assert any(l for _, _, l in v2) and any(not l for _, _, l in v2)
+    import ctypes as C
+
+    L = rsf_amd.lib()
+    n = C.c_int()
+    assert L.rsf_variant_count(3, C.byref(n)) == rsf_amd.ERR_INVALID_ARGUMENT
+    assert L.rsf_variant_info(1, len(v1), None, 0, None, None) == rsf_amd.ERR_INVALID_ARGUMENT
+
+
+def test_device_wrappers_refuse_host_and_mismatched_tensors():
+    """The device-pointer wrappers hand tensor addresses to kernels: host
+    tensors, numpy arrays, wrong dtypes, non-contiguous views and tensors
+    smaller than the blocks are refused before any library call."""
+    import torch
+
+    chk = rsf_amd._device_tensor
+    with pytest.raises(TypeError):
+        chk(np.zeros(8, np.float32), "phi")
+    with pytest.raises(ValueError):
+        chk(torch.zeros(8), "phi")           # a host tensor
+
+    class FakeDev:
+        """Stands in for a device tensor (no HIP device here)."""
+        def __init__(self, t):
+            self.t, self.is_cuda, self.dtype, self.shape, self.device = t, True, t.dtype, t.shape, "cuda:0"
+
+        def data_ptr(self):
+            return 0
+
+        def is_contiguous(self):
+            return self.t.is_contiguous()
+
+        def numel(self):
+            return self.t.numel()
+
+    chk(FakeDev(torch.zeros(2, 3, 4)), "phi", numel=24, shape=(2, 3, 4))
+    with pytest.raises(TypeError):
+        chk(FakeDev(torch.zeros(8, dtype=torch.float64)), "phi")
+    with pytest.raises(ValueError):
+        chk(FakeDev(torch.zeros(4, 6).t()), "phi")         # not contiguous
+    with pytest.raises(ValueError):
+        chk(FakeDev(torch.zeros(20)), "phi", numel=24)      # smaller than the blocks
+    with pytest.raises(ValueError):
+        chk(FakeDev(torch.zeros(2, 4, 3)), "phi", shape=(2, 3, 4))
+    chk(FakeDev(torch.zeros(8, dtype=torch.uint8)), "image", dtypes=rsf_amd._IMAGE_DTYPES)
+    assert rsf_amd._extent([(0, 4, 4, 4), (64, 2, 2, 2)]) == 72
+
+
+def test_workspace_size_counts_the_split_sweep_buffer():
+    """rsf_workspace_size of a plain run: phi ping-pong, transform and force, plus
+    the split sweep-1 xy channels (4 literal, 2 fast); fast mode adds blur(I), the
+    float image and two pass temporaries; blocks of 2^30 voxels and more run the
+    fused sweep 1 without the xy buffer; tap counts without a fused kernel take the
+    unfused pipeline's 22 volumes."""
+    import ctypes as C
+
+    L = rsf_amd.lib()
+
+    def ws(mode, w, h, l, sigma=3.0):
+        b = C.c_int64()
+        rsf_amd.check(L.rsf_workspace_size(C.byref(rsf_amd.make_params(sigma=sigma, mode=mode)), w, h, l,
+                                           C.byref(b)))
+        return b.value
+
+    n = 64 * 70 * 80
+    assert ws(rsf_amd.MODE_LITERAL, 64, 70, 80) == (3 + 4) * 4 * n
+    assert ws(rsf_amd.MODE_FAST, 64, 70, 80) == (3 + 2 + 2) * 4 * n + 2 * 4 * (64 - 20) * 70 * 80
+    big = (1024, 1024, 1024)
+    assert ws(rsf_amd.MODE_LITERAL, *big) == 3 * 4 * 1024 ** 3
+    assert ws(rsf_amd.MODE_LITERAL, 64, 70, 80, sigma=5.3) == 22 * 4 * n
+    b = C.c_int64()
+    assert L.rsf_workspace_size(C.byref(rsf_amd.make_params()), 10, 70, 80, C.byref(b)) == rsf_amd.ERR_INVALID_ARGUMENT
