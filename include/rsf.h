@@ -112,6 +112,18 @@ int rsf_workspace_size(const rsf_params* p, int64_t w, int64_t h, int64_t l, int
  * phi_in and phi_out must not overlap. */
 int rsf_step_device(rsf_context* ctx, const void* image, int image_dtype, const float* phi_in,
                     float* phi_out, int64_t w, int64_t h, int64_t l, const rsf_params* p);
+/* rsf_step_device without any device allocation -- no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its workspace is a
+ * static device array of the library (68 MiB per device the library runs on,
+ * held until the process ends): the transform and force of at most 2^23
+ * voxels (e.g. 203^3) plus the tile plan.  Larger volumes return
+ * RSF_ERR_INVALID_ARGUMENT.  LITERAL mode with sigma2 = 0 only (else
+ * RSF_ERR_UNSUPPORTED); same bits as rsf_step_device.  Every call on one
+ * device shares that device's array: calls that overlap in time on one device
+ * (two host threads, or two contexts' streams) race and give wrong results. */
+int rsf_step_device_noalloc(rsf_context* ctx, const void* image, int image_dtype, const float* phi_in,
+                            float* phi_out, int64_t w, int64_t h, int64_t l, const rsf_params* p);
 /* p->iterations steps in place on phi -- the loop of CUDA_RSF/main.cpp:222-314,
  * captured in a hipGraph.  iter_ms (NULL or p->iterations floats) receives
  * the per-iteration device time, the figure main.cpp:309 prints.  After the

@@ -14,6 +14,7 @@
 #include <list>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -55,9 +56,10 @@ int DevBuf::ensure(size_t n) {
 }
 
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = true;
 }
 
 // --------------------------------------------------------------- parameters
@@ -165,9 +167,43 @@ struct rsf_context {
     const void* held_image = nullptr;
     std::string held_key;
     DevBuf nonfinite;  // count of NaN/inf voxels found after a run
+    unsigned char* scratch = nullptr;  // this device's copy of the allocation-free step's scratch
 };
 
+// ------------------------------------------------- allocation-free step scratch
+// rsf_step_device_noalloc keeps its workspace here instead of allocating it:
+// the transform T(phi) and the force F of up to kNoAllocMaxVoxels voxels, then
+// the tile plan (tile lists, sweep-2 flags, block table).  Each device the
+// library runs on holds its own copy for the life of the process.  In host code
+// `scratch` only names the symbol; its device address comes from
+// hipGetSymbolAddress, once per device (scratch_address).
 namespace {
+constexpr int64_t kNoAllocMaxVoxels = int64_t(1) << 23;
+constexpr size_t kNoAllocPlanBytes = size_t(4) << 20;
+constexpr size_t kNoAllocBytes = 2 * (size_t)kNoAllocMaxVoxels * sizeof(float) + kNoAllocPlanBytes;
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_addr;   // device -> address of its scratch
+
+// The device address of `scratch` on the current device `device` (looked up once).
+int scratch_address(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) {
+        *out = it->second;
+        return RSF_OK;
+    }
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    g_scratch_addr[device] = static_cast<unsigned char*>(ptr);
+    *out = static_cast<unsigned char*>(ptr);
+    return RSF_OK;
+}
 
 int ensure_ws(rsf_context* c, DevBuf& b, size_t bytes) {
     if (b.p && b.bytes >= bytes) return RSF_OK;
@@ -459,20 +495,38 @@ int prepare_image(rsf_context* c, const Prepared& pr, const void* image, int u8)
 
 // One iteration phi_in -> phi_out over the prepared blocks (tr must hold the
 // transform of phi_in; it is overwritten with the transform of phi_out).
+// One sigma2 = 0 iteration on explicit buffers: tr holds T(phi_in) and gets
+// T(phi_out); F is the force; blurI the image blur (fast mode only).
+int enqueue_step_plain(const TilePlan& tp, const Derived& d, const void* image, int u8, const float* blurI,
+                       float* tr, float* F, const float* phi_in, float* phi_out, const rsf_params* p,
+                       hipStream_t s) {
+    SweepArgs a{};
+    a.tr = tr;
+    a.img = image;
+    a.img_u8 = u8;
+    a.blurI = blurI;
+    a.out0 = F;
+    a.out_fits = 0;
+    RSF_TRY(launch_sweep1(tp, a, d, p, s));
+    a.phi = phi_in;
+    a.F = F;
+    a.phi_out = phi_out;
+    a.tr_out = tr;
+    return launch_sweep2(tp, a, p, s);
+}
+
 int enqueue_step(rsf_context* c, const Prepared& pr, const void* image, int u8, const float* phi_in,
                  float* phi_out, const rsf_params* p) {
     const TilePlan& tp = pr.runner->plan;
+    if (pr.sw == 0)
+        return enqueue_step_plain(tp, pr.d, image, u8, c->blurI.as<float>(), c->tr.as<float>(), c->F.as<float>(),
+                                  phi_in, phi_out, p, c->stream);
     SweepArgs a{};
     a.tr = c->tr.as<float>();
     a.img = image;
     a.img_u8 = u8;
     a.blurI = c->blurI.as<float>();
-    if (pr.sw == 0) {
-        a.out0 = c->F.as<float>();
-        a.out1 = nullptr;
-        a.out_fits = 0;
-        RSF_TRY(launch_sweep1(tp, a, pr.d, p, c->stream));
-    } else {
+    {
         a.out0 = c->fo.as<float>();
         a.out1 = c->fi.as<float>();
         a.out_fits = 1;
@@ -701,6 +755,7 @@ int rsf_context_create(int device, void* hip_stream, rsf_context** out) {
     c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     RSF_HIP(hipEventCreate(&c->ev0));
     RSF_HIP(hipEventCreate(&c->ev1));
+    RSF_TRY(scratch_address(device, &c->scratch));   // with `device` current
     load_tune_file(c.get());
     *out = c.release();
     return RSF_OK;
@@ -807,6 +862,64 @@ int rsf_step_device(rsf_context* c, const void* image, int image_dtype, const fl
     if (phi_in == phi_out) return set_error(RSF_ERR_INVALID_ARGUMENT, "phi_in and phi_out must not alias");
     rsf_block b{0, w, h, l};
     return rsf_step_blocks_device(c, &b, 1, image, image_dtype, phi_in, phi_out, p);
+}
+
+int rsf_step_device_noalloc(rsf_context* c, const void* image, int image_dtype, const float* phi_in,
+                            float* phi_out, int64_t w, int64_t h, int64_t l, const rsf_params* p) {
+    RSF_TRY(check_ctx(c));
+    RSF_TRY(check_dtype(image_dtype));
+    if (!image || !phi_in || !phi_out) return set_error(RSF_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (!c->scratch) return set_error(RSF_ERR_DEVICE, "no scratch address for device %d", c->device);
+    Derived d;
+    RSF_TRY(derive(p, &d));
+    if (p->mode != RSF_MODE_LITERAL || (int)p->sigma2 != 0)
+        return set_error(RSF_ERR_UNSUPPORTED, "the allocation-free step runs LITERAL mode with sigma2 = 0");
+    if (w < d.P || h < d.P || l < d.P)
+        return set_error(RSF_ERR_INVALID_ARGUMENT, "dims (%lld,%lld,%lld) are smaller than the %d-tap blur",
+                         (long long)w, (long long)h, (long long)l, d.P);
+    if (w > kNoAllocMaxVoxels || h > kNoAllocMaxVoxels || l > kNoAllocMaxVoxels || w * h * l > kNoAllocMaxVoxels)
+        return set_error(RSF_ERR_INVALID_ARGUMENT,
+                         "%lld x %lld x %lld voxels exceed the allocation-free step's scratch of %lld voxels",
+                         (long long)w, (long long)h, (long long)l, (long long)kNoAllocMaxVoxels);
+    const int64_t n = w * h * l;
+    if (ranges_overlap(phi_in, phi_out, n)) return set_error(RSF_ERR_INVALID_ARGUMENT, "phi_in and phi_out overlap");
+    const std::vector<BlockDesc> blocks = {BlockDesc{0, (int)w, (int)h, (int)l, 0}};
+    int s1, s2, zc1, zc2;
+    pick_variants(c, blocks, d, 1, aligned16({phi_in, phi_out}), &s1, &s2, &zc1, &zc2);
+    if (s1 >= 0 && sweep1_variants()[s1].fn2) {
+        // the fused sweep 1 keeps its xy-blurred channels on chip: no intermediate volume
+        s1 = -1;
+        for (size_t k = 0; k < sweep1_variants().size(); ++k) {
+            const auto& v = sweep1_variants()[k];
+            if (!v.fn2 && v.P == d.P && v.literal) {
+                s1 = (int)k;
+                break;
+            }
+        }
+        if (s1 >= 0) {
+            const auto& v = sweep1_variants()[s1];
+            const int64_t cols = ((w + v.tx - 1) / v.tx) * ((h + v.ty - 1) / v.ty);
+            zc1 = (int)l;
+            while (zc1 > 4 * d.P && cols * ((l + zc1 - 1) / zc1) < 2 * (int64_t)c->num_cus) zc1 = (zc1 + 1) / 2;
+        }
+    }
+    if (s1 < 0 || s2 < 0)
+        return set_error(RSF_ERR_UNSUPPORTED, "tap count %d has no fused kernel for the allocation-free step", d.P);
+    TilePlan tp;
+    RSF_TRY(plan_tiles_host(tp, blocks, s1, s2, zc1, zc2, c->num_cus));
+    // scratch layout: T(phi) | F | tile plan, each on a 256-byte boundary
+    const size_t vol = ((size_t)n * sizeof(float) + 255) & ~size_t(255);
+    float* tr = reinterpret_cast<float*>(c->scratch);
+    float* F = reinterpret_cast<float*>(c->scratch + vol);
+    unsigned char* arena = c->scratch + 2 * vol;
+    const size_t arena_bytes = kNoAllocBytes - 2 * vol;
+    const hipStream_t s = c->stream;
+    RSF_TRY(plan_upload(tp, arena, arena_bytes, s));
+    RSF_TRY(launch_transform(tr, phi_in, n, 1, p->epsilon, s));
+    RSF_TRY(enqueue_step_plain(tp, d, image, image_dtype == RSF_DTYPE_U8, nullptr, tr, F, phi_in, phi_out, p, s));
+    // also keeps the plan's host tile lists alive until their copies have run
+    RSF_HIP(hipStreamSynchronize(s));
+    return RSF_OK;
 }
 
 int rsf_step_blocks_device(rsf_context* c, const rsf_block* blocks, int nblocks, const void* image,

@@ -31,11 +31,18 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // Validates a context, makes its device current and returns its stream.
 int context_stream(rsf_context* c, hipStream_t* s);
 
-// Grow-only device allocation.
+// Grow-only device allocation, or (bind) a view of memory someone else owns.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = true;
     int ensure(size_t n);
+    void bind(void* q, size_t n) {
+        release();
+        p = q;
+        bytes = n;
+        owned = false;
+    }
     void release();
     ~DevBuf() { release(); }
     template <class T> T* as() const { return static_cast<T*>(p); }
@@ -144,6 +151,14 @@ struct TilePlan {
 
 int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s2, int zc1,
                int zc2, int num_cus);
+// plan_tiles in two halves: the tile lists and grids on the host, then the
+// device copies.  plan_upload with an arena places every device buffer of the
+// plan (tiles, flags, block table, split-sweep xy channels) inside it, aligned
+// to 256 bytes, and copies on stream s; plan_arena_bytes(tp) is what it needs.
+int plan_tiles_host(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s2, int zc1, int zc2,
+                    int num_cus);
+size_t plan_arena_bytes(const TilePlan& tp);
+int plan_upload(TilePlan& tp, unsigned char* arena, size_t arena_bytes, hipStream_t s);
 int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, const rsf_params* p,
                   hipStream_t s);
 int launch_sweep2(const TilePlan& tp, const SweepArgs& a, const rsf_params* p, hipStream_t s);

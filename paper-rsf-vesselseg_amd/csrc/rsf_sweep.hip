@@ -1617,8 +1617,8 @@ static void xcd_order(std::vector<Tile>& t) {
     t.swap(out);
 }
 
-int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s2, int zc1, int zc2,
-               int num_cus) {
+int plan_tiles_host(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s2, int zc1, int zc2,
+                    int num_cus) {
     const auto& V1 = sweep1_variants();
     const auto& V2 = sweep2_variants();
     if (s1 < 0 || s1 >= (int)V1.size() || s2 < 0 || s2 >= (int)V2.size())
@@ -1697,33 +1697,79 @@ int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s
     if (split1) {
         tp.grid1 = tp.tiles1.empty() ? 0 : grid_of(tp.tiles1.size(), 4);
         tp.grid5 = tp.tiles5.empty() ? 0 : grid_of(tp.tiles5.size(), 8);
-        RSF_TRY(tp.xy.ensure((size_t)tp.extent * (V1[s1].literal ? 4 : 2) * sizeof(float)));
     }
-    RSF_TRY(tp.d_tiles1.ensure(tp.tiles1.size() * sizeof(Tile)));
-    RSF_TRY(tp.d_tiles2.ensure(tp.tiles2.size() * sizeof(Tile)));
-    RSF_TRY(tp.d_blocks.ensure(blocks.size() * sizeof(BlockDesc)));
-    RSF_HIP(hipMemcpy(tp.d_tiles1.p, tp.tiles1.data(), tp.tiles1.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    RSF_HIP(hipMemcpy(tp.d_tiles2.p, tp.tiles2.data(), tp.tiles2.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    if (V2[s2].fix_fn && !tp.tiles2.empty()) {
-        // every launch of the interior rewrites its flags; zero them for a clean start
-        const size_t nf = tp.tiles2.size() * (size_t)(V2[s2].tx * V2[s2].ty / 64) * sizeof(int);
-        RSF_TRY(tp.d_flags2.ensure(nf));
-        RSF_HIP(hipMemset(tp.d_flags2.p, 0, nf));
-    }
-    if (!tp.tiles3.empty()) {
-        RSF_TRY(tp.d_tiles3.ensure(tp.tiles3.size() * sizeof(Tile)));
-        RSF_HIP(hipMemcpy(tp.d_tiles3.p, tp.tiles3.data(), tp.tiles3.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    }
-    if (!tp.tiles5.empty()) {
-        RSF_TRY(tp.d_tiles5.ensure(tp.tiles5.size() * sizeof(Tile)));
-        RSF_HIP(hipMemcpy(tp.d_tiles5.p, tp.tiles5.data(), tp.tiles5.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    }
-    if (!tp.tiles4.empty()) {
-        RSF_TRY(tp.d_tiles4.ensure(tp.tiles4.size() * sizeof(Tile)));
-        RSF_HIP(hipMemcpy(tp.d_tiles4.p, tp.tiles4.data(), tp.tiles4.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    }
-    RSF_HIP(hipMemcpy(tp.d_blocks.p, blocks.data(), blocks.size() * sizeof(BlockDesc), hipMemcpyHostToDevice));
     return RSF_OK;
+}
+
+namespace {
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+size_t flags_bytes(const TilePlan& tp) {
+    const Sweep2Variant& v = sweep2_variants()[tp.s2];
+    return (v.fix_fn && !tp.tiles2.empty()) ? tp.tiles2.size() * (size_t)(v.tx * v.ty / 64) * sizeof(int) : 0;
+}
+size_t xy_bytes(const TilePlan& tp) {
+    const Sweep1Variant& v = sweep1_variants()[tp.s1];
+    return v.fn2 ? (size_t)tp.extent * (v.literal ? 4 : 2) * sizeof(float) : 0;
+}
+}  // namespace
+
+size_t plan_arena_bytes(const TilePlan& tp) {
+    return align256(tp.tiles1.size() * sizeof(Tile)) + align256(tp.tiles2.size() * sizeof(Tile)) +
+           align256(tp.tiles3.size() * sizeof(Tile)) + align256(tp.tiles4.size() * sizeof(Tile)) +
+           align256(tp.tiles5.size() * sizeof(Tile)) + align256(tp.blocks.size() * sizeof(BlockDesc)) +
+           align256(flags_bytes(tp)) + align256(xy_bytes(tp));
+}
+
+int plan_upload(TilePlan& tp, unsigned char* arena, size_t arena_bytes, hipStream_t s) {
+    const size_t nf = flags_bytes(tp), nxy = xy_bytes(tp);
+    struct Part {
+        DevBuf* buf;
+        const void* src;
+        size_t bytes;
+    };
+    const Part parts[] = {
+        {&tp.d_tiles1, tp.tiles1.data(), tp.tiles1.size() * sizeof(Tile)},
+        {&tp.d_tiles2, tp.tiles2.data(), tp.tiles2.size() * sizeof(Tile)},
+        {&tp.d_tiles3, tp.tiles3.data(), tp.tiles3.size() * sizeof(Tile)},
+        {&tp.d_tiles4, tp.tiles4.data(), tp.tiles4.size() * sizeof(Tile)},
+        {&tp.d_tiles5, tp.tiles5.data(), tp.tiles5.size() * sizeof(Tile)},
+        {&tp.d_blocks, tp.blocks.data(), tp.blocks.size() * sizeof(BlockDesc)},
+        {&tp.d_flags2, nullptr, nf},
+        {&tp.xy, nullptr, nxy},
+    };
+    if (arena) {
+        // every buffer a view of the caller's arena; copies and clears on stream s
+        if (plan_arena_bytes(tp) > arena_bytes)
+            return set_error(RSF_ERR_INVALID_ARGUMENT, "tile plan needs %zu bytes of scratch, %zu available",
+                             plan_arena_bytes(tp), arena_bytes);
+        size_t off = 0;
+        for (const Part& q : parts) {
+            if (q.bytes == 0) {
+                q.buf->release();
+                continue;
+            }
+            q.buf->bind(arena + off, q.bytes);
+            if (q.src) RSF_HIP(hipMemcpyAsync(q.buf->p, q.src, q.bytes, hipMemcpyHostToDevice, s));
+            else if (q.buf == &tp.d_flags2) RSF_HIP(hipMemsetAsync(q.buf->p, 0, q.bytes, s));
+            off += align256(q.bytes);
+        }
+        return RSF_OK;
+    }
+    // owned buffers (grow-only), synchronous copies
+    for (const Part& q : parts) {
+        if (q.bytes == 0) continue;
+        RSF_TRY(q.buf->ensure(q.bytes));
+        if (q.src) RSF_HIP(hipMemcpy(q.buf->p, q.src, q.bytes, hipMemcpyHostToDevice));
+    }
+    // the interior rewrites its flags on every launch; zero them for a clean start
+    if (nf) RSF_HIP(hipMemset(tp.d_flags2.p, 0, nf));
+    return RSF_OK;
+}
+
+int plan_tiles(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, int s2, int zc1, int zc2,
+               int num_cus) {
+    RSF_TRY(plan_tiles_host(tp, blocks, s1, s2, zc1, zc2, num_cus));
+    return plan_upload(tp, nullptr, 0, nullptr);
 }
 
 int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, const rsf_params* p,

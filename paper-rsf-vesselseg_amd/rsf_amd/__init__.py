@@ -94,6 +94,7 @@ _SIGS = {
     "rsf_context_release_image": (_I, [_P]),
     "rsf_workspace_size": (_I, [_PP, _I64, _I64, _I64, C.POINTER(_I64)]),
     "rsf_step_device": (_I, [_P, _P, _I, _P, _P, _I64, _I64, _I64, _PP]),
+    "rsf_step_device_noalloc": (_I, [_P, _P, _I, _P, _P, _I64, _I64, _I64, _PP]),
     "rsf_run_device": (_I, [_P, _P, _I, _P, _I64, _I64, _I64, _PP, _P]),
     "rsf_run_host": (_I, [_P, _P, _I, _P, _I64, _I64, _I64, _PP, _P]),
     "rsf_last_loop_ms": (_I, [_P, C.POINTER(_F)]),
