@@ -23,9 +23,16 @@ Also reported on the same line:
   batched       64 uneven blocks (sides from {96,160,192,256}, seed 1000+b),
                 300 iterations: one persistent launch per sweep per
                 iteration in a graph, vs the same blocks one after another;
+  adjoint       the differentiated batch: 8 x 128^3 (seeds 4000+b), 32 unrolled
+                steps with checkpoints every 8, BCE loss; forward and backward;
+  large         1024x1024x768 (2000 tubes, radius 1.5-8, noise 20, seed 5) for
+                --steps iterations with the default kernels and with the
+                autotuned ones;
   cpu_baseline  the oracle (CPU restatement of the reference pipeline,
                 unfused, FP32) on the same 512^3 volume for a bounded number
-                of iterations on this host's cores.
+                of iterations on this host's cores, one thread on a 256^3 crop,
+                and the 128^3 CPU configuration (40 tubes, seed 1) run in full
+                for 100 iterations; `host` is lscpu's description of the CPU.
 """
 import argparse
 import json
@@ -355,7 +362,52 @@ def adjoint_run(rsf_amd, torch, ctx, K=32, C=8, nvol=8, n=128):
     }
 
 
-def cpu_baseline(img, phi0, budget_s):
+def host_cpu():
+    """The host's CPU as lscpu reports it (model name, sockets, cores), read before
+    anything touches the GPU; /proc/cpuinfo fields as a fallback."""
+    import subprocess
+
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() in ("Model name", "Vendor ID", "CPU family", "Model", "Stepping", "Socket(s)",
+                             "Core(s) per socket", "Thread(s) per core", "CPU max MHz", "L3 cache"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if "Model name" not in info:
+        try:
+            with open("/proc/cpuinfo") as f:
+                info["Model name"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                                          "")
+        except OSError:
+            pass
+    info["model_name"] = info.get("Model name", "")
+    return info
+
+
+def config1_cpu(rsf_amd, threads):
+    """The CPU configuration run in full: 128^3 uint8 (40 tubes, seed 1), sigma 3, lambda 1,
+    nu 65.025, mu 1, dt 0.1, eps 1, 100 iterations of the oracle on `threads` host threads."""
+    import numpy as np
+
+    import oracle
+
+    img, _ = rsf_amd.synth_tubes((128, 128, 128), tubes=40, rmin=2.0, rmax=6.0, bias=20.0, noise=15.0, seed=1)
+    phi0 = rsf_amd.initial_phi(img, 1.0, 110.0)
+    p = oracle.make_params(sigma=3.0, lambda1=1.0, nu=65.025, mu=1.0, dt=0.1, epsilon=1.0)
+    t0 = time.perf_counter()
+    phi = oracle.run(phi0, img.astype(np.float32), p, 100, threads)
+    t = time.perf_counter() - t0
+    log(f"cpu oracle, config 1: 100 iterations of 128^3 in {t:.1f} s on {threads} threads")
+    return {"workload": "128^3 uint8, 40 tubes (seed 1), 100 iterations, run in full", "seconds": round(t, 2),
+            "value": round(128 ** 3 * 100 / t / 1e9, 5), "unit": "Gvoxel-iterations/s", "cores": threads,
+            "inside_fraction": round(float((phi < 0).mean()), 5)}
+
+
+def cpu_baseline(img, phi0, budget_s, rsf_amd=None, host=None):
     import numpy as np
 
     import oracle
@@ -383,13 +435,8 @@ def cpu_baseline(img, phi0, budget_s):
     oracle.run(cphi, cI, p, 1, 1)
     t_one = time.perf_counter() - t0
     log(f"cpu oracle, 1 thread: 1 iteration of {c[2]}x{c[1]}x{c[0]} in {t_one:.1f} s")
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
-    return {
+    host = host or host_cpu()
+    out = {
         "value": round(n * iters / t1 / 1e9, 5),
         "unit": "Gvoxel-iterations/s",
         "cores": threads,
@@ -398,8 +445,11 @@ def cpu_baseline(img, phi0, budget_s):
                   f"restatement of CUDA_RSF/main.cpp:234-300, OpenMP)",
         "one_thread": {"value": round(cphi.size / t_one / 1e9, 5), "unit": "Gvoxel-iterations/s",
                        "sample": f"1 iteration of a {c[2]}x{c[1]}x{c[0]} crop of the same workload"},
-        "host": {"nproc": os.cpu_count(), "model_name": model},
+        "host": host,
     }
+    if rsf_amd is not None:
+        out["config1_full_run"] = config1_cpu(rsf_amd, threads)
+    return out
 
 
 def traffic_from_profiles(kernel, mode):
@@ -427,11 +477,13 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
-    ap.add_argument("--large", action="store_true",
-                    help="also run the 1024x1024x768 volume (2000 tubes), default and autotuned kernels")
-    ap.add_argument("--large-steps", type=int, default=1000)
-    ap.add_argument("--adjoint", action="store_true",
-                    help="also time the differentiated batch (8 x 128^3, 32 steps, checkpoints every 8)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 1024x1024x768 volume (2000 tubes), default and autotuned kernels")
+    ap.add_argument("--large", action="store_true", help="(the default; kept for old commands)")
+    ap.add_argument("--large-steps", type=int, default=None, help="iterations of the large volume (default --steps)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiated batch (8 x 128^3, 32 steps, checkpoints every 8)")
+    ap.add_argument("--adjoint", action="store_true", help="(the default; kept for old commands)")
     ap.add_argument("--unet", action="store_true",
                     help="also time the U-Net seed producer's tiled inference on 128x512x512")
     ap.add_argument("--io", action="store_true",
@@ -439,6 +491,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scale for rocprofv3 byte counters)")
     args = ap.parse_args()
+    host = host_cpu() if not args.no_cpu else None   # lscpu runs before the GPU is initialised
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the RSF engine runs each volume or batch of blocks on a single "
@@ -529,16 +582,16 @@ def main():
             "parity": "one-step gate met (max|dphi| <= 2e-5 max|phi|); full-run sign masks within FP32 "
                       "noise, not bit-identical",
         }
-    if args.adjoint:
+    if not args.no_adjoint:
         line["adjoint"] = adjoint_run(rsf_amd, torch, ctx)
-    if args.large:
-        line["large"] = large_run(rsf_amd, torch, ctx, args.large_steps, mode)
+    if not args.no_large:
+        line["large"] = large_run(rsf_amd, torch, ctx, args.large_steps or args.steps, mode)
     if args.io:
         line["io"] = io_run(rsf_amd, torch, ctx)
     if args.unet:
         line["unet"] = unet_run(rsf_amd, torch, ctx)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(r["img"], r["phi0"], args.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(r["img"], r["phi0"], args.cpu_budget, rsf_amd, host)
     ctx.close()
     print(json.dumps(line), flush=True)
 

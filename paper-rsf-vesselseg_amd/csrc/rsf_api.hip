@@ -439,6 +439,11 @@ int prepare(rsf_context* c, const rsf_block* blocks, int nblocks, const rsf_para
     return st;
 }
 
+// true when the plan's sweep 1 is the split xy/z pair (which can also blur the image)
+bool split_plan(const Prepared& pr) {
+    return pr.fused && sweep1_variants()[pr.runner->plan.s1].fn2 != nullptr;
+}
+
 int ensure_workspace(rsf_context* c, const Prepared& pr) {
     const size_t n = (size_t)pr.extent * sizeof(float);
     RSF_TRY(ensure_ws(c, c->phi2, n));
@@ -452,7 +457,8 @@ int ensure_workspace(rsf_context* c, const Prepared& pr) {
         RSF_TRY(ensure_ws(c, c->Ei, n));
     }
     if (!pr.literal || pr.sw > 0) RSF_TRY(ensure_ws(c, c->imgf, n));
-    if (!pr.literal) {
+    if (!pr.literal && !split_plan(pr)) {
+        // pass temporaries of the per-op image blur (a split plan blurs through its own kernels)
         size_t tmax = 0;
         for (auto& b : pr.blocks) tmax = std::max(tmax, (size_t)(b.w - pr.d.P + 1) * b.h * b.l);
         RSF_TRY(ensure_ws(c, c->tmp1, tmax * sizeof(float)));
@@ -484,17 +490,24 @@ int prepare_image(rsf_context* c, const Prepared& pr, const void* image, int u8)
         if (key == c->held_key) return RSF_OK;
         c->held_key.clear();
     }
-    RSF_TRY(ops::to_float(c->imgf.as<float>(), image, u8, pr.extent, c->stream));
-    if (!pr.literal)
+    const bool split = split_plan(pr);
+    if (pr.sw > 0 || (!pr.literal && !split))
+        RSF_TRY(ops::to_float(c->imgf.as<float>(), image, u8, pr.extent, c->stream));
+    if (!pr.literal && split) {
+        // blur(I) through the plan's own xy and z kernels: with T = 1 their channel 0
+        // is the image (the transform buffer is free until the caller's launch_transform)
+        RSF_HIP(hipMemsetD32Async((hipDeviceptr_t)c->tr.p, 0x3f800000, (size_t)pr.extent, c->stream));
+        RSF_TRY(launch_blur_image(pr.runner->plan, image, u8, c->tr.as<float>(), c->blurI.as<float>(), pr.d,
+                                  c->stream));
+    } else if (!pr.literal) {
         for (auto& b : pr.blocks)
             RSF_TRY(ops::blur_pad(c->blurI.as<float>() + b.off, c->imgf.as<float>() + b.off, c->tmp1.as<float>(),
                                   c->tmp2.as<float>(), b.w, b.h, b.l, pr.d.taps, pr.d.P, pr.d.m, c->stream));
+    }
     if (!key.empty()) c->held_key = key;
     return RSF_OK;
 }
 
-// One iteration phi_in -> phi_out over the prepared blocks (tr must hold the
-// transform of phi_in; it is overwritten with the transform of phi_out).
 // One sigma2 = 0 iteration on explicit buffers: tr holds T(phi_in) and gets
 // T(phi_out); F is the force; blurI the image blur (fast mode only).
 int enqueue_step_plain(const TilePlan& tp, const Derived& d, const void* image, int u8, const float* blurI,
@@ -515,6 +528,8 @@ int enqueue_step_plain(const TilePlan& tp, const Derived& d, const void* image, 
     return launch_sweep2(tp, a, p, s);
 }
 
+// One iteration phi_in -> phi_out over the prepared blocks (tr must hold the
+// transform of phi_in; it is overwritten with the transform of phi_out).
 int enqueue_step(rsf_context* c, const Prepared& pr, const void* image, int u8, const float* phi_in,
                  float* phi_out, const rsf_params* p) {
     const TilePlan& tp = pr.runner->plan;
@@ -823,8 +838,9 @@ int rsf_workspace_size(const rsf_params* p, int64_t w, int64_t h, int64_t l, int
     bool split = false;
     for (const auto& v : sweep1_variants()) split = split || (v.fn2 && v.P == d.P && v.literal == literal);
     if (small && split) vols += literal ? 4 : 2;
-    // fast mode: the two pass temporaries of the image blur
-    const int64_t tmp = literal ? 0 : 2 * (w - d.P + 1) * h * l;
+    // fast mode without a split sweep 1: the two pass temporaries of the per-op image
+    // blur (fewer bytes than the split kernels' two xy channels)
+    const int64_t tmp = (literal || (small && split)) ? 0 : 2 * (w - d.P + 1) * h * l;
     *bytes = (vols * n + tmp) * (int64_t)sizeof(float);
     return RSF_OK;
 }

@@ -571,8 +571,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 // ring over the xy-blurred channels; every voxel outside the valid region is
 // in the pad band (fits 30/30 = 1).  U8: image dtype; MODE 0 writes F with
 // lambda1 == lambda2, 3 F with separate lambdas, 1 the fits, 2 the fits and
-// the blurred Heavisides (the lambda form compiled in keeps the plane loop one
-// basic block).  All accesses are buffer
+// the blurred Heavisides, 4 the padded blur of channel 0 (the lambda form
+// compiled in keeps the plane loop one basic block).  All accesses are buffer
 // operations: the column's offset in a VGPR, the plane's offset in an SGPR
 // (blocks stay below 2^32 bytes per buffer).
 template <int P, bool LIT, int TXC, int TYC, int D, bool U8, int MODE>
@@ -621,6 +621,11 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         };
         // ring outputs d[ch] (valid) or the pad band; bI: image blur (fast mode)
         auto store_out = [&](int z, float Iv, bool valid, float bI, const float (&d)[CH]) {
+            if constexpr (MODE == 4) {
+                // the padded blur of channel 0 itself (pad value 30, kernel.cu:313)
+                st(r0, z, valid ? d[0] : 30.0f);
+                return;
+            }
             float fo = 1.0f, fi = 1.0f, bH = 30.0f, bHc = 30.0f;  // pad band (kernel.cu:313, main.cpp:277)
             if (valid) {
                 if constexpr (LIT) {
@@ -645,7 +650,7 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         };
         auto band = [&](int z) {
             const float zero[CH] = {};
-            store_out(z, ldi(z), false, 0.f, zero);
+            store_out(z, MODE == 4 ? 0.f : ldi(z), false, 0.f, zero);
         };
         const bool xyvalid = x >= M && x < w - M && y >= M && y < h - M;
         if (!xyvalid) {
@@ -663,6 +668,9 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         // The plane loop is unrolled by P (a multiple of D): ring and queue slots
         // are fixed.
         float qc[D][CH], qi[D], qB[D];
+        if constexpr (MODE == 4)
+#pragma unroll
+            for (int k = 0; k < D; ++k) qi[k] = qB[k] = 0.f;
         auto group = [&](int t0, auto guard_) __attribute__((always_inline)) {
             constexpr bool GUARD = decltype(guard_)::value;
 #pragma unroll
@@ -679,8 +687,9 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
                     for (int ch = 0; ch < CH; ++ch) qc[k][ch] = ldf(rC[ch], tn);
                     if (!GUARD || tn >= P - 1) {
-                        qi[k] = ldi(tn - (P - 1) + M);
-                        if constexpr (!LIT) qB[k] = ldf(rL, tn - (P - 1) + M);
+                        // blur mode (4) reads neither the image nor an image blur
+                        if constexpr (MODE != 4) qi[k] = ldi(tn - (P - 1) + M);
+                        if constexpr (!LIT && MODE != 4) qB[k] = ldf(rL, tn - (P - 1) + M);
                     }
                 }
                 // plane t adds tap g[(s - j) mod P] to accumulator j and completes
@@ -1502,6 +1511,8 @@ Sweep1Variant s1split(const void* xy_fn, const char* name) {
     v.fn2s[5] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 2>;
     v.fn2s[6] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 3>;
     v.fn2s[7] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 3>;
+    v.fn2s[8] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 4>;
+    v.fn2s[9] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 4>;
     return v;
 }
 
@@ -1805,6 +1816,29 @@ int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, cons
                     (void*)&tp.d_tiles1.p, (void*)&ntiles, (void*)&tp.zc1,   (void*)&d.taps,
                     (void*)&d.halfS3,   (void*)&p->lambda1, (void*)&p->lambda2, (void*)&same};
     RSF_HIP(hipLaunchKernel(v.fn, dim3(tp.grid1), dim3(v.nt), args, 0, s));
+    return RSF_OK;
+}
+
+int launch_blur_image(const TilePlan& tp, const void* img, int u8, const float* ones, float* out, const Derived& d,
+                      hipStream_t s) {
+    const Sweep1Variant& v = sweep1_variants()[tp.s1];
+    if (!v.fn2 || v.P != d.P) return set_error(RSF_ERR_UNSUPPORTED, "image blur needs a split sweep-1 plan");
+    const int ntiles = (int)tp.tiles1.size(), n5 = (int)tp.tiles5.size();
+    float* xy = tp.xy.as<float>();
+    const int64_t stride = tp.extent;
+    const int zero = 0;
+    const float fzero = 0.f;
+    const float* null = nullptr;
+    float* nullo = nullptr;
+    if (ntiles > 0) {
+        void* a1[] = {(void*)&ones, (void*)&img, (void*)&u8, (void*)&xy, (void*)&stride, (void*)&tp.d_blocks.p,
+                      (void*)&tp.d_tiles1.p, (void*)&ntiles, (void*)&tp.zc1, (void*)&d.taps};
+        RSF_HIP(hipLaunchKernel(v.fn_img[u8] ? v.fn_img[u8] : v.fn, dim3(tp.grid1), dim3(v.nt), a1, 0, s));
+    }
+    void* a2[] = {(void*)&xy,    (void*)&stride, (void*)&img,   (void*)&null,  (void*)&out,
+                  (void*)&nullo, (void*)&nullo,  (void*)&nullo, (void*)&tp.d_blocks.p, (void*)&tp.d_tiles5.p,
+                  (void*)&n5,    (void*)&d.taps, (void*)&fzero, (void*)&fzero, (void*)&fzero, (void*)&zero};
+    RSF_HIP(hipLaunchKernel(v.fn2s[8 + u8], dim3(tp.grid5), dim3(v.tx2 * v.ty2), a2, 0, s));
     return RSF_OK;
 }
 

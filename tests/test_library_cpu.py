@@ -283,8 +283,8 @@ def test_device_wrappers_refuse_host_and_mismatched_tensors():
 
 def test_workspace_size_counts_the_split_sweep_buffer():
     """rsf_workspace_size of a plain run: phi ping-pong, transform and force, plus
-    the split sweep-1 xy channels (4 literal, 2 fast); fast mode adds blur(I), the
-    float image and two pass temporaries; blocks of 2^30 voxels and more run the
+    the split sweep-1 xy channels (4 literal, 2 fast); fast mode adds blur(I) and
+    the float image, and two pass temporaries where no split kernel blurs the image; blocks of 2^30 voxels and more run the
     fused sweep 1 without the xy buffer; tap counts without a fused kernel take the
     unfused pipeline's 22 volumes."""
     import ctypes as C
@@ -299,7 +299,9 @@ def test_workspace_size_counts_the_split_sweep_buffer():
 
     n = 64 * 70 * 80
     assert ws(rsf_amd.MODE_LITERAL, 64, 70, 80) == (3 + 4) * 4 * n
-    assert ws(rsf_amd.MODE_FAST, 64, 70, 80) == (3 + 2 + 2) * 4 * n + 2 * 4 * (64 - 20) * 70 * 80
+    assert ws(rsf_amd.MODE_FAST, 64, 70, 80) == (3 + 2 + 2) * 4 * n
+    # no split kernel at this tap count: the per-op image blur's two pass temporaries
+    assert ws(rsf_amd.MODE_FAST, 64, 70, 80, sigma=2.5) == (3 + 2) * 4 * n + 2 * 4 * (64 - 16) * 70 * 80
     big = (1024, 1024, 1024)
     assert ws(rsf_amd.MODE_LITERAL, *big) == 3 * 4 * 1024 ** 3
     assert ws(rsf_amd.MODE_LITERAL, 64, 70, 80, sigma=5.3) == 22 * 4 * n
