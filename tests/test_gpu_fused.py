@@ -241,7 +241,9 @@ def test_kernel_variants_give_identical_results(ctx, dtype, mode, lambdas, shape
     xy/z) and each sweep-2 kernel (vector, scalar buffer, direct, LDS), the set
     rsf_autotune chooses from -- computes the same operations in the same order:
     a run gives bit-identical phi whichever variants and z chunks it takes, with
-    equal lambdas and with separate ones (whose force forms differ).  The second
+    equal lambdas and with separate ones (whose force forms differ).  The fast
+    mode's matrix-core xy kernels sum the blur in another order: they are held
+    to the fast-mode gate against the oracle (3 steps within 1e-4 max|phi|).  The second
     shape has full and partial tiles and rows that are not 16-byte aligned (w =
     130), the first aligned rows (the vector sweep-2 interior).  In literal mode
     the default run is also the oracle's phi bit for bit."""
@@ -267,11 +269,21 @@ def test_kernel_variants_give_identical_results(ctx, dtype, mode, lambdas, shape
         torch.cuda.synchronize()
         results[(c1, c2)] = d_phi.cpu().numpy()
     ref = results[(None, None)]
+    want = oracle.run(phi0, img.astype(np.float32), oracle_params(p), 3)
     for k, v in results.items():
+        if k[0] is not None and "mfma" in k[0]:
+            # the matrix-core blurs sum in their own order (fast mode only): the
+            # fast-mode gate against the oracle instead of bit equality
+            assert mode == rsf_amd.MODE_FAST
+            err, _ = one_step_gate(v, want)
+            assert err <= 1e-4 * float(np.abs(want).max()), (k, err)
+            continue
         assert np.array_equal(v, ref), (k, float(np.abs(v - ref).max()))
     if mode == rsf_amd.MODE_LITERAL:
-        want = oracle.run(phi0, img.astype(np.float32), oracle_params(p), 3)
         assert np.array_equal(ref, want), float(np.abs(ref - want).max())
+    else:
+        err, _ = one_step_gate(ref, want)
+        assert err <= 1e-4 * float(np.abs(want).max()), err
 
 
 @pytest.mark.parametrize("P", list(range(7, 37, 2)))
