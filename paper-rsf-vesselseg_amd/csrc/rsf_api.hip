@@ -471,8 +471,9 @@ int ensure_workspace(rsf_context* c, const Prepared& pr) {
 // whatever preparation the held image had there is gone.
 void invalidate_held(rsf_context* c) { c->held_key.clear(); }
 
-// Once per run: image as float and, in fast mode, the padded blur of the image.
-int prepare_image(rsf_context* c, const Prepared& pr, const void* image, int u8) {
+// Once per run: image as float and, in fast mode, the padded blur of the image
+// (c->tr is scratch meanwhile).
+int prepare_image_only(rsf_context* c, const Prepared& pr, const void* image, int u8) {
     if (pr.literal && pr.sw == 0) return RSF_OK;
     // a held image keeps its preparation while the layout, taps, mode and workspace match
     std::string key;
@@ -494,9 +495,7 @@ int prepare_image(rsf_context* c, const Prepared& pr, const void* image, int u8)
     if (pr.sw > 0 || (!pr.literal && !split))
         RSF_TRY(ops::to_float(c->imgf.as<float>(), image, u8, pr.extent, c->stream));
     if (!pr.literal && split) {
-        // blur(I) through the plan's own xy and z kernels: with T = 1 their channel 0
-        // is the image (the transform buffer is free until the caller's launch_transform)
-        RSF_HIP(hipMemsetD32Async((hipDeviceptr_t)c->tr.p, 0x3f800000, (size_t)pr.extent, c->stream));
+        // blur(I) through the plan's own xy and z kernels
         RSF_TRY(launch_blur_image(pr.runner->plan, image, u8, c->tr.as<float>(), c->blurI.as<float>(), pr.d,
                                   c->stream));
     } else if (!pr.literal) {
@@ -506,6 +505,12 @@ int prepare_image(rsf_context* c, const Prepared& pr, const void* image, int u8)
     }
     if (!key.empty()) c->held_key = key;
     return RSF_OK;
+}
+
+// Once per run: the image's preparation, then the transform T(phi) into c->tr.
+int prepare_image(rsf_context* c, const Prepared& pr, const void* image, int u8, const float* phi, float eps) {
+    RSF_TRY(prepare_image_only(c, pr, image, u8));
+    return launch_transform(c->tr.as<float>(), phi, pr.extent, pr.literal, eps, c->stream);
 }
 
 // One sigma2 = 0 iteration on explicit buffers: tr holds T(phi_in) and gets
@@ -592,11 +597,33 @@ int capture(rsf_context* c, const Prepared& pr, const void* image, int u8, float
 __global__ void k_nonfinite(const float* __restrict__ phi, const BlockDesc* __restrict__ blocks, int nblocks,
                             unsigned long long* __restrict__ count) {
     unsigned long long bad = 0;
+    auto nf = [](float v) { return (__float_as_uint(v) & 0x7f800000u) == 0x7f800000u ? 1u : 0u; };
     for (int b = blockIdx.y; b < nblocks; b += gridDim.y) {
         const BlockDesc bd = blocks[b];
         const int64_t n = (int64_t)bd.w * bd.h * bd.l;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-            bad += (__float_as_uint(phi[bd.off + i]) & 0x7f800000u) == 0x7f800000u;
+        const float* p = phi + bd.off;
+        const int64_t step = (int64_t)gridDim.x * blockDim.x;
+        int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        int64_t done = 0;
+        if (((uintptr_t)p & 15) == 0) {
+            // 16-byte loads, four in flight per thread
+            const float4* p4 = reinterpret_cast<const float4*>(p);
+            const int64_t n4 = n >> 2;
+            int64_t j = i;
+            for (; j + 3 * step < n4; j += 4 * step) {
+                float4 v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[k] = p4[j + k * step];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) bad += nf(v[k].x) + nf(v[k].y) + nf(v[k].z) + nf(v[k].w);
+            }
+            for (; j < n4; j += step) {
+                const float4 v = p4[j];
+                bad += nf(v.x) + nf(v.y) + nf(v.z) + nf(v.w);
+            }
+            done = n4 << 2;
+        }
+        for (i += done; i < n; i += step) bad += nf(p[i]);
     }
     if (bad) atomicAdd(count, bad);
 }
@@ -607,7 +634,7 @@ int check_finite(rsf_context* c, const Prepared& pr, const float* phi) {
     RSF_HIP(hipMemsetAsync(c->nonfinite.p, 0, sizeof(unsigned long long), c->stream));
     int64_t nmax = 1;
     for (auto& b : pr.blocks) nmax = std::max(nmax, (int64_t)b.w * b.h * b.l);
-    const dim3 grid((unsigned)std::min<int64_t>((nmax + 255) / 256, 1024),
+    const dim3 grid((unsigned)std::min<int64_t>((nmax + 255) / 256, 2048),
                     (unsigned)std::min<size_t>(pr.blocks.size(), 65535));
     hipLaunchKernelGGL(k_nonfinite, grid, dim3(256), 0, c->stream, phi, pr.runner->plan.d_blocks.as<BlockDesc>(),
                        (int)pr.blocks.size(), c->nonfinite.as<unsigned long long>());
@@ -622,8 +649,7 @@ int check_finite(rsf_context* c, const Prepared& pr, const float* phi) {
 int run_prepared(rsf_context* c, const Prepared& pr, const void* image, int u8, float* phi, const rsf_params* p,
                  float* iter_ms) {
     RSF_TRY(ensure_workspace(c, pr));
-    RSF_TRY(prepare_image(c, pr, image, u8));
-    RSF_TRY(launch_transform(c->tr.as<float>(), phi, pr.extent, pr.literal, p->epsilon, c->stream));
+    RSF_TRY(prepare_image(c, pr, image, u8, phi, p->epsilon));
     const int T = p->iterations;
     if (T == 0) return RSF_OK;
     Runner* r = pr.runner;
@@ -960,8 +986,7 @@ int rsf_step_blocks_device(rsf_context* c, const rsf_block* blocks, int nblocks,
         return RSF_OK;
     }
     RSF_TRY(ensure_workspace(c, pr));
-    RSF_TRY(prepare_image(c, pr, image, u8));
-    RSF_TRY(launch_transform(c->tr.as<float>(), phi_in, pr.extent, pr.literal, p->epsilon, c->stream));
+    RSF_TRY(prepare_image(c, pr, image, u8, phi_in, p->epsilon));
     RSF_TRY(enqueue_step(c, pr, image, u8, phi_in, phi_out, p));
     RSF_HIP(hipStreamSynchronize(c->stream));
     return RSF_OK;
@@ -1041,8 +1066,7 @@ int rsf_profile_sweeps(rsf_context* c, const rsf_block* blocks, int nblocks, con
     if (!pr.fused || pr.sw != 0) return set_error(RSF_ERR_UNSUPPORTED, "profiling covers the fused sigma2 = 0 path");
     const int u8 = image_dtype == RSF_DTYPE_U8;
     RSF_TRY(ensure_workspace(c, pr));
-    RSF_TRY(prepare_image(c, pr, image, u8));
-    RSF_TRY(launch_transform(c->tr.as<float>(), phi, pr.extent, pr.literal, p->epsilon, c->stream));
+    RSF_TRY(prepare_image(c, pr, image, u8, phi, p->epsilon));
     const int T = std::max(1, (int)p->iterations);
     std::vector<hipEvent_t> ev(3 * T);
     for (auto& e : ev) RSF_HIP(hipEventCreate(&e));
@@ -1227,8 +1251,7 @@ int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks
     float *fo = A[0], *fi = A[1], *PH = A[2], *PHc = A[3], *QH = A[4], *QA = A[5], *Bh = A[6], *Ba = A[7];
     float *Gx = A[8], *Gy = A[9], *Gz = A[10], *t1 = A[11], *t2 = A[12], *parts = A[13];
     hipStream_t s = c->stream;
-    RSF_TRY(prepare_image(c, pr, image, u8));
-    RSF_TRY(launch_transform(c->tr.as<float>(), phi, pr.extent, pr.literal, p->epsilon, s));
+    RSF_TRY(prepare_image(c, pr, image, u8, phi, p->epsilon));
     SweepArgs a{};
     a.tr = c->tr.as<float>();
     a.img = image;

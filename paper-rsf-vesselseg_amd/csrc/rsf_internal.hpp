@@ -117,8 +117,10 @@ struct Sweep1Variant {
     const char* name;
     const void* fn2;       // split variants: the z kernel (else null)
     int tx2, ty2;          // z-kernel columns per work item
+    int nt2;               // z-kernel threads per workgroup (sweep modes)
     const void* fn2s[10];  // z kernel by 2 * output mode + (image is uint8)
     const void* fn_img[2]; // split variants: the xy kernel by (image is uint8), else null
+    const void* fn_blur[2];// split fast variants: the image-only xy kernel by (image is uint8), else null
 };
 struct Sweep2Variant {
     int literal;
@@ -162,11 +164,12 @@ int plan_upload(TilePlan& tp, unsigned char* arena, size_t arena_bytes, hipStrea
 int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, const rsf_params* p,
                   hipStream_t s);
 int launch_sweep2(const TilePlan& tp, const SweepArgs& a, const rsf_params* p, hipStream_t s);
-// out = the image blurred x -> y -> z (valid correlation, kernel.cu:1000-1049) and
-// padded with 30 (kernel.cu:300-325), through the split sweep-1 kernels of the
-// plan; `ones` is a float volume of 1.0 over the plan's extent (channel 0 of
-// the xy kernel is then the image itself).  RSF_ERR_UNSUPPORTED for a fused plan.
-int launch_blur_image(const TilePlan& tp, const void* img, int u8, const float* ones, float* out, const Derived& d,
+// Fast mode, once per run: out = the image blurred x -> y -> z (valid
+// correlation, kernel.cu:1000-1049) and padded with 30 (kernel.cu:300-325),
+// through the split sweep-1 kernels of the plan (its image-only xy kernel, or
+// for variants without one the xy kernel on T = 1 written into tr, which is then
+// scratch).  RSF_ERR_UNSUPPORTED for a fused or literal plan.
+int launch_blur_image(const TilePlan& tp, const void* img, int u8, float* tr, float* out, const Derived& d,
                       hipStream_t s);
 // tr = H(phi) (literal) or G(phi) (fast) on n voxels
 int launch_transform(float* tr, const float* phi, int64_t n, int literal, float eps, hipStream_t s);

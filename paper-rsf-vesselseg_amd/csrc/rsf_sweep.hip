@@ -345,17 +345,21 @@ __device__ __forceinline__ void sweep1_channels(float i, float t, float (&c)[LIT
 // IMG: the image dtype fixed at compile time (1 uint8, 0 float32; -1 takes the
 // img_u8 argument), so that a uint8 image's prefetch keeps one packed word per
 // four voxels in registers.
-template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, bool HI = false, int RX = 4, int UX = 1,
-          int UY = 1, int IMG = -1>
+// ONLY (fast mode): the xy blur of the image alone, into channel 0 (tr is not
+// read): the first half of the padded image blur the fast fits need.
+template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int HI = 0, int RX = 4, int UX = 1,
+          int UY = 1, int IMG = -1, bool ONLY = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_kernel(
     const float* __restrict__ tr, const void* __restrict__ img, int img_u8_arg, float* __restrict__ xy,
     int64_t stride, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc,
     Taps taps) {
     const int img_u8 = IMG >= 0 ? IMG : img_u8_arg;
     // HI (literal mode): LDS holds H and I, and each x-pass item forms a pair
-    // of channels from them -- (I*H, H) or (I*(1-H), 1-H) -- before blurring,
-    // which halves the LDS input tile.
+    // of channels from them -- (I*H, H) or (I*(1-H), 1-H) -- before blurring
+    // (HI = 1), or all four (HI = 2: one window read and one 1-H per element
+    // for four channels), which halves the LDS input tile.
     static_assert(!HI || LIT, "HI is a literal-mode layout");
+    static_assert(!ONLY || !LIT, "the image-only blur is a fast-mode pass");
     constexpr int CH = LIT ? 4 : 2;
     constexpr int CIN = HI ? 2 : CH;       // LDS input arrays
     constexpr int M = (P - 1) / 2;
@@ -370,10 +374,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     // UX, UY: unroll of the x- and y-pass item loops.  The LDS footprint caps the
     // kernel at two workgroups per CU, so registers are free to hold the next
     // item's loads in flight while the current item computes.
-    constexpr int NXI = (HI ? 2 : CH) * RH * XG;  // x-pass work items
-    constexpr int NYI = CH * TX * (TY / R);  // y-pass work items
+    // x- and y-pass work items; the image-only blur has one channel, and its
+    // loops stop at the item count (whole idle waves in the last round)
+    constexpr int NXI = ONLY || HI == 2 ? RH * XG : (HI ? 2 : CH) * RH * XG;
+    constexpr int NYI = ONLY ? TX * (TY / R) : CH * TX * (TY / R);
+    constexpr int NXR = (NXI + NT - 1) / NT, NYR = (NYI + NT - 1) / NT;
     constexpr int WIN = (RX + P - 1 + 3) & ~3;
-    static_assert(NXI % NT == 0 && NYI % NT == 0 && TY % R == 0, "work items must be multiples of the threads");
+    static_assert(ONLY || HI == 2 || (NXI % NT == 0 && NYI % NT == 0), "work items must be multiples of the threads");
+    static_assert(TY % R == 0, "y-pass rows");
     __shared__ __align__(16) float sIn[CIN][RH][RWP];
     // x-pass output: channel planes CS floats apart, CS = TX (mod 64), so the
     // groups of TX lanes of a wave in the y pass (same rows, neighbouring
@@ -406,7 +414,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 const int gx = x0 - M + 4 * q;
                 const int64_t row = zoff + (int64_t)gy * w;
                 if (vec && gx + 3 < w) {
-                    pt[k] = *reinterpret_cast<const float4*>(tr + row + gx);
+                    if constexpr (!ONLY) pt[k] = *reinterpret_cast<const float4*>(tr + row + gx);
                     if (img_u8) {
                         pi[k].x = __uint_as_float(*reinterpret_cast<const uint32_t*>(img8 + row + gx));
                     } else {
@@ -417,7 +425,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int64_t idx = row + min(gx + j, w - 1);
-                        tv[j] = tr[idx];
+                        tv[j] = ONLY ? 0.f : tr[idx];
                         iv[j] = img_u8 ? (float)img8[idx] : imgf[idx];
                     }
                     pt[k] = make_float4(tv[0], tv[1], tv[2], tv[3]);
@@ -443,7 +451,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 } else {
                     iv[0] = pi[k].x; iv[1] = pi[k].y; iv[2] = pi[k].z; iv[3] = pi[k].w;
                 }
-                if constexpr (HI) {
+                if constexpr (ONLY) {
+                    *reinterpret_cast<float4*>(&sIn[0][r][4 * q]) = make_float4(iv[0], iv[1], iv[2], iv[3]);
+                } else if constexpr (HI) {
                     *reinterpret_cast<float4*>(&sIn[0][r][4 * q]) = pt[k];
                     *reinterpret_cast<float4*>(&sIn[1][r][4 * q]) = make_float4(iv[0], iv[1], iv[2], iv[3]);
                 } else {
@@ -464,7 +474,47 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             if (z + 1 < zend) fetch(z + 1);
             __syncthreads();
             // ---- x pass (one item at a time: registers, not latency, are the limit)
-            if constexpr (HI) {
+            if constexpr (HI == 2) {
+#pragma unroll UX
+                for (int k = 0; k < NXR; ++k) {
+                    const int e = tid + k * NT;
+                    if (NXI % NT != 0 && e >= NXI) break;
+                    const int r = e / XG;
+                    const int xb = (e - r * XG) * RX;
+                    float cw[4][WIN];
+                    const float4* sh = reinterpret_cast<const float4*>(&sIn[0][r][xb]);
+                    const float4* si = reinterpret_cast<const float4*>(&sIn[1][r][xb]);
+#pragma unroll
+                    for (int q = 0; q < WIN / 4; ++q) {
+                        const float4 h4 = sh[q], i4 = si[q];
+                        const float hv[4] = {h4.x, h4.y, h4.z, h4.w}, iv[4] = {i4.x, i4.y, i4.z, i4.w};
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            // the channel values of sweep1_channels
+                            const float hc = 1.0f - hv[j];
+                            cw[0][4 * q + j] = iv[j] * hv[j];
+                            cw[1][4 * q + j] = hv[j];
+                            cw[2][4 * q + j] = iv[j] * hc;
+                            cw[3][4 * q + j] = hc;
+                        }
+                    }
+#pragma unroll
+                    for (int ch = 0; ch < 4; ++ch) {
+                        float oc[RX];
+#pragma unroll
+                        for (int o = 0; o < RX; ++o) {
+                            float a = cw[ch][o] * taps.g[0];
+#pragma unroll
+                            for (int kk = 1; kk < P; ++kk) a = fmaf(cw[ch][o + kk], taps.g[kk], a);
+                            oc[o] = a;
+                        }
+#pragma unroll
+                        for (int o = 0; o < RX; o += 4)
+                            *reinterpret_cast<float4*>(&sX[ch * CS + r * TXP + xb + o]) =
+                                make_float4(oc[o], oc[o + 1], oc[o + 2], oc[o + 3]);
+                    }
+                }
+            } else if constexpr (HI) {
 #pragma unroll UX
                 for (int k = 0; k < NXI / NT; ++k) {
                     const int e = tid + k * NT;
@@ -509,8 +559,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 }
             } else
 #pragma unroll UX
-            for (int k = 0; k < NXI / NT; ++k) {
+            for (int k = 0; k < NXR; ++k) {
                 const int e = tid + k * NT;
+                if (NXI % NT != 0 && e >= NXI) break;
                 const int ch = e / (RH * XG);
                 const int rem = e - ch * (RH * XG);
                 const int r = rem / XG;
@@ -541,12 +592,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             __syncthreads();
             // ---- y pass: R outputs of one column, rows r0 .. r0+R-1
 #pragma unroll UY
-            for (int k = 0; k < NYI / NT; ++k) {
+            for (int k = 0; k < NYR; ++k) {
                 // item e: column cx = e % TX, then channel, then row group, so that
                 // with TX = 32 the two halves of a wave take two channels of the same rows
                 const int e = tid + k * NT;
+                if (NYI % NT != 0 && e >= NYI) break;
                 const int cx = e % TX, t = e / TX;
-                const int ch = t % CH, r0 = (t / CH) * R;
+                const int ch = ONLY ? 0 : t % CH, r0 = (ONLY ? t : t / CH) * R;
                 float col[R + P - 1];
 #pragma unroll
                 for (int j = 0; j < R + P - 1; ++j) col[j] = sX[ch * CS + (r0 + j) * TXP + cx];
@@ -769,6 +821,7 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
     float* __restrict__ out2, float* __restrict__ out3, const BlockDesc* __restrict__ blocks,
     const Tile* __restrict__ tiles, int ntiles, Taps taps, float halfS3, float l1, float l2, int same_lambda) {
     constexpr int CH = LIT ? 4 : 2;
+    constexpr int CHL = MODE == 4 ? 1 : CH;   // channels read and blurred
     constexpr int M = (P - 1) / 2;
     constexpr int ESZ = U8 ? 1 : 4;
     static_assert(P % D == 0, "P must be a multiple of the load queue depth");
@@ -845,16 +898,16 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
             continue;
         }
         for (int z = 0; z < M; ++z) band(z);
-        float acc[CH][P];
+        float acc[CHL][P];
 #pragma unroll
-        for (int ch = 0; ch < CH; ++ch)
+        for (int ch = 0; ch < CHL; ++ch)
 #pragma unroll
             for (int j = 0; j < P; ++j) acc[ch][j] = 0.f;
         // D planes of loads in flight: the channels of plane t and the image (and,
         // fast mode, the image blur) of the plane it completes, o = t - (P-1) + M.
         // The plane loop is unrolled by P (a multiple of D): ring and queue slots
         // are fixed.
-        float qc[D][CH], qi[D], qB[D];
+        float qc[D][CHL], qi[D], qB[D];
         if constexpr (MODE == 4)
 #pragma unroll
             for (int k = 0; k < D; ++k) qi[k] = qB[k] = 0.f;
@@ -867,12 +920,12 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
                 const int k = s % D;
                 float v[CH];
 #pragma unroll
-                for (int ch = 0; ch < CH; ++ch) v[ch] = qc[k][ch];
+                for (int ch = 0; ch < CHL; ++ch) v[ch] = qc[k][ch];
                 const float vi = qi[k], vB = qB[k];
                 const int tn = t + D;
                 if (!GUARD || tn < l) {
 #pragma unroll
-                    for (int ch = 0; ch < CH; ++ch) qc[k][ch] = ldf(rC[ch], tn);
+                    for (int ch = 0; ch < CHL; ++ch) qc[k][ch] = ldf(rC[ch], tn);
                     if (!GUARD || tn >= P - 1) {
                         // blur mode (4) reads neither the image nor an image blur
                         if constexpr (MODE != 4) qi[k] = ldi(tn - (P - 1) + M);
@@ -885,12 +938,12 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
                 for (int j = 0; j < P; ++j) {
                     const float g = taps.g[(s - j + P) % P];
 #pragma unroll
-                    for (int ch = 0; ch < CH; ++ch) acc[ch][j] = fmaf(v[ch], g, acc[ch][j]);
+                    for (int ch = 0; ch < CHL; ++ch) acc[ch][j] = fmaf(v[ch], g, acc[ch][j]);
                 }
                 const int js = (s + 1) % P;
-                float d[CH];
+                float d[CH] = {};
 #pragma unroll
-                for (int ch = 0; ch < CH; ++ch) {
+                for (int ch = 0; ch < CHL; ++ch) {
                     d[ch] = acc[ch][js];
                     acc[ch][js] = 0.f;
                 }
@@ -902,7 +955,7 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         for (int k = 0; k < D; ++k) {
             if (k < l) {
 #pragma unroll
-                for (int ch = 0; ch < CH; ++ch) qc[k][ch] = ldf(rC[ch], k);
+                for (int ch = 0; ch < CHL; ++ch) qc[k][ch] = ldf(rC[ch], k);
             }
         }
         group(0, std::true_type{});
@@ -1903,9 +1956,24 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
 
 #pragma clang fp contract(fast)
 
-__global__ void transform_kernel(float* tr, const float* phi, int64_t n, int lit, float eps, float inv_eps) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        tr[i] = lit ? heav_literal(phi[i], eps) : heav_centred(phi[i], inv_eps);
+// T(phi) over n voxels: four per thread (one 16-byte load and store) where both
+// buffers are 16-byte aligned (V4), then the tail one per thread.
+template <bool LIT, bool V4>
+__global__ void transform_kernel(float* __restrict__ tr, const float* __restrict__ phi, int64_t n, float eps,
+                                 float inv_eps) {
+    auto T = [&](float p) { return LIT ? heav_literal(p, eps) : heav_centred(p, inv_eps); };
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t done = 0;
+    if constexpr (V4) {
+        const int64_t n4 = n >> 2;
+        for (int64_t j = i; j < n4; j += step) {
+            const float4 p = reinterpret_cast<const float4*>(phi)[j];
+            reinterpret_cast<float4*>(tr)[j] = make_float4(T(p.x), T(p.y), T(p.z), T(p.w));
+        }
+        done = n4 << 2;
+    }
+    for (int64_t j = done + i; j < n; j += step) tr[j] = T(phi[j]);
 }
 
 template <int P, bool LIT, int TX, int TY, int NT, int WPE = 0, bool DB = false>
@@ -1955,6 +2023,7 @@ template <int P, bool LIT, int TX, int TY, int NT, int TXC, int TYC, int D>
 Sweep1Variant s1split(const void* xy_fn, const char* name) {
     Sweep1Variant v{P, LIT ? 1 : 0, TX, TY, NT, xy_fn, name,
                     (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0>, TXC, TYC, {}};
+    v.nt2 = TXC * TYC;
     v.fn2s[0] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 0>;
     v.fn2s[1] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0>;
     v.fn2s[2] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 1>;
@@ -1968,13 +2037,17 @@ Sweep1Variant s1split(const void* xy_fn, const char* name) {
     return v;
 }
 
-template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D, bool HI = false,
+template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D, int HI = 0,
           int RX = 4, int UX = 1, int UY = 1>
 Sweep1Variant s1s(const char* name) {
     Sweep1Variant v = s1split<P, LIT, TX, TY, NT, TXC, TYC, D>(
         (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1>, name);
     v.fn_img[0] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 0>;
     v.fn_img[1] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1>;
+    if constexpr (!LIT) {
+        v.fn_blur[0] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 0, true>;
+        v.fn_blur[1] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1, true>;
+    }
     return v;
 }
 template <int P, int TX, int NT, int WPE, int TXC, int TYC, int D>
@@ -2005,6 +2078,10 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, true, 16, 108, 256, 9, 2, 256, 1, 3, true, 8>("p21_lit_hi_16x108x8"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true, 8>("p21_lit_hi_32x44x8"),
+        // all four channels per x-pass item from one H/I window (2.22 ms at 512^3
+        // against 2.24 for the default: within the run-to-run spread)
+        s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, 2>("p21_lit_q4_32x60r15"),
+        s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, 2, 8>("p21_lit_q4_32x44x8"),
         s1s<21, true, 32, 28, 256, 7, 4, 256, 1, 3>("p21_lit_split_32x28w4"),
 
         s1s<21, true, 64, 28, 512, 7, 4, 256, 1, 3>("p21_lit_split_64x28w4"),
@@ -2276,7 +2353,7 @@ int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, cons
                       (void*)&tp.d_blocks.p, (void*)&tp.d_tiles5.p, (void*)&n5, (void*)&d.taps,
                       (void*)&d.halfS3, (void*)&p->lambda1, (void*)&p->lambda2, (void*)&same};
         const int zmode = a.out_fits == 0 && !same ? 3 : a.out_fits;
-        RSF_HIP(hipLaunchKernel(v.fn2s[2 * zmode + a.img_u8], dim3(tp.grid5), dim3(v.tx2 * v.ty2), a2, 0, s));
+        RSF_HIP(hipLaunchKernel(v.fn2s[2 * zmode + a.img_u8], dim3(tp.grid5), dim3(v.nt2), a2, 0, s));
         return RSF_OK;
     }
     void* args[] = {(void*)&a.tr,       (void*)&a.img,     (void*)&a.img_u8, (void*)&a.blurI,
@@ -2288,10 +2365,10 @@ int launch_sweep1(const TilePlan& tp, const SweepArgs& a, const Derived& d, cons
     return RSF_OK;
 }
 
-int launch_blur_image(const TilePlan& tp, const void* img, int u8, const float* ones, float* out, const Derived& d,
+int launch_blur_image(const TilePlan& tp, const void* img, int u8, float* tr, float* out, const Derived& d,
                       hipStream_t s) {
     const Sweep1Variant& v = sweep1_variants()[tp.s1];
-    if (!v.fn2 || v.P != d.P) return set_error(RSF_ERR_UNSUPPORTED, "image blur needs a split sweep-1 plan");
+    if (!v.fn2 || v.P != d.P || v.literal) return set_error(RSF_ERR_UNSUPPORTED, "image blur needs a split fast plan");
     const int ntiles = (int)tp.tiles1.size(), n5 = (int)tp.tiles5.size();
     float* xy = tp.xy.as<float>();
     const int64_t stride = tp.extent;
@@ -2300,9 +2377,15 @@ int launch_blur_image(const TilePlan& tp, const void* img, int u8, const float* 
     const float* null = nullptr;
     float* nullo = nullptr;
     if (ntiles > 0) {
-        void* a1[] = {(void*)&ones, (void*)&img, (void*)&u8, (void*)&xy, (void*)&stride, (void*)&tp.d_blocks.p,
+        const void* fn = v.fn_blur[u8];
+        if (!fn) {
+            // a variant without the image-only xy kernel blurs (I*1, 1): channel 0 is the image
+            RSF_HIP(hipMemsetD32Async((hipDeviceptr_t)tr, 0x3f800000, (size_t)tp.extent, s));
+            fn = v.fn_img[u8] ? v.fn_img[u8] : v.fn;
+        }
+        void* a1[] = {(void*)&tr, (void*)&img, (void*)&u8, (void*)&xy, (void*)&stride, (void*)&tp.d_blocks.p,
                       (void*)&tp.d_tiles1.p, (void*)&ntiles, (void*)&tp.zc1, (void*)&d.taps};
-        RSF_HIP(hipLaunchKernel(v.fn_img[u8] ? v.fn_img[u8] : v.fn, dim3(tp.grid1), dim3(v.nt), a1, 0, s));
+        RSF_HIP(hipLaunchKernel(fn, dim3(tp.grid1), dim3(v.nt), a1, 0, s));
     }
     void* a2[] = {(void*)&xy,    (void*)&stride, (void*)&img,   (void*)&null,  (void*)&out,
                   (void*)&nullo, (void*)&nullo,  (void*)&nullo, (void*)&tp.d_blocks.p, (void*)&tp.d_tiles5.p,
@@ -2341,8 +2424,13 @@ int launch_sweep2(const TilePlan& tp, const SweepArgs& a, const rsf_params* p, h
 
 int launch_transform(float* tr, const float* phi, int64_t n, int literal, float eps, hipStream_t s) {
     if (n <= 0) return RSF_OK;
-    int64_t g = std::min<int64_t>((n + 255) / 256, 1 << 20);
-    hipLaunchKernelGGL(transform_kernel, dim3((unsigned)g), dim3(256), 0, s, tr, phi, n, literal, eps, 1.0f / eps);
+    const bool v4 = (((uintptr_t)tr | (uintptr_t)phi) & 15) == 0;
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>((v4 ? n / 4 + 255 : n + 255) / 256, 1 << 20));
+    const void* fn = literal ? (v4 ? (const void*)&transform_kernel<true, true> : (const void*)&transform_kernel<true, false>)
+                             : (v4 ? (const void*)&transform_kernel<false, true> : (const void*)&transform_kernel<false, false>);
+    const float inv_eps = 1.0f / eps;
+    void* args[] = {(void*)&tr, (void*)&phi, (void*)&n, (void*)&eps, (void*)&inv_eps};
+    RSF_HIP(hipLaunchKernel(fn, dim3((unsigned)g), dim3(256), args, 0, s));
     RSF_HIP(hipGetLastError());
     return RSF_OK;
 }
