@@ -332,6 +332,21 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
             }
         }
     }
+    // a z kernel over the interleaved intermediate addresses P + D planes of it
+    // through 32-bit offsets: larger planes take the first variant without that limit
+    if (*s1 >= 0 && sweep1_variants()[*s1].max_plane > 0) {
+        int64_t pmax = 0;
+        for (const auto& b : blocks) pmax = std::max<int64_t>(pmax, (int64_t)b.w * b.h);
+        if (pmax > sweep1_variants()[*s1].max_plane) {
+            for (size_t k = 0; k < sweep1_variants().size(); ++k) {
+                const auto& v = sweep1_variants()[k];
+                if (v.P == d.P && v.literal == literal && v.max_plane == 0) {
+                    *s1 = (int)k;
+                    break;
+                }
+            }
+        }
+    }
     // The split and vector sweep-2 kernels address a block through 32-bit buffer
     // offsets; blocks of 2^30 voxels and more take the direct kernel.
     bool small = true;

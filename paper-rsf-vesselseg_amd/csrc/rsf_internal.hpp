@@ -118,6 +118,7 @@ struct Sweep1Variant {
     const void* fn2;       // split variants: the z kernel (else null)
     int tx2, ty2;          // z-kernel columns per work item
     int nt2;               // z-kernel threads per workgroup (sweep modes)
+    int64_t max_plane;     // largest plane (w*h) its z kernel addresses, 0: any
     const void* fn2s[10];  // z kernel by 2 * output mode + (image is uint8)
     const void* fn_img[2]; // split variants: the xy kernel by (image is uint8), else null
     const void* fn_blur[2];// split fast variants: the image-only xy kernel by (image is uint8), else null

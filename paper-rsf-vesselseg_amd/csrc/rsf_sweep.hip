@@ -347,8 +347,13 @@ __device__ __forceinline__ void sweep1_channels(float i, float t, float (&c)[LIT
 // four voxels in registers.
 // ONLY (fast mode): the xy blur of the image alone, into channel 0 (tr is not
 // read): the first half of the padded image blur the fast fits need.
+// IL (literal mode): the output interleaved per voxel, channels 0..3 of voxel v
+// at xy[4v .. 4v+3]; each y-pass item blurs one channel pair (0,1) or (2,3) of a
+// column and stores it as one 8-byte word, and the two halves of a wave take the
+// two pairs of the same columns, so a wave's stores cover whole 16-byte voxels.
+// (The fast kernel has no registers to spare for it: its variants spilled.)
 template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int HI = 0, int RX = 4, int UX = 1,
-          int UY = 1, int IMG = -1, bool ONLY = false>
+          int UY = 1, int IMG = -1, bool ONLY = false, bool IL = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_kernel(
     const float* __restrict__ tr, const void* __restrict__ img, int img_u8_arg, float* __restrict__ xy,
     int64_t stride, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc,
@@ -360,6 +365,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     // for four channels), which halves the LDS input tile.
     static_assert(!HI || LIT, "HI is a literal-mode layout");
     static_assert(!ONLY || !LIT, "the image-only blur is a fast-mode pass");
+    static_assert(!IL || (LIT && TX == 32 && !ONLY), "the interleaved layout: literal, 32-column tiles");
     constexpr int CH = LIT ? 4 : 2;
     constexpr int CIN = HI ? 2 : CH;       // LDS input arrays
     constexpr int M = (P - 1) / 2;
@@ -377,7 +383,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     // x- and y-pass work items; the image-only blur has one channel, and its
     // loops stop at the item count (whole idle waves in the last round)
     constexpr int NXI = ONLY || HI == 2 ? RH * XG : (HI ? 2 : CH) * RH * XG;
-    constexpr int NYI = ONLY ? TX * (TY / R) : CH * TX * (TY / R);
+    constexpr int NYI = ONLY ? TX * (TY / R) : (IL ? CH / 2 : CH) * TX * (TY / R);
     constexpr int NXR = (NXI + NT - 1) / NT, NYR = (NYI + NT - 1) / NT;
     constexpr int WIN = (RX + P - 1 + 3) & ~3;
     static_assert(ONLY || HI == 2 || (NXI % NT == 0 && NYI % NT == 0), "work items must be multiples of the threads");
@@ -385,8 +391,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     __shared__ __align__(16) float sIn[CIN][RH][RWP];
     // x-pass output: channel planes CS floats apart, CS = TX (mod 64), so the
     // groups of TX lanes of a wave in the y pass (same rows, neighbouring
-    // channels) read disjoint LDS banks
-    constexpr int CS0 = RH * TXP, CS = CS0 + (((TX % 64 - CS0 % 64) % 64) + 64) % 64;
+    // channels) read disjoint LDS banks; in the interleaved y pass the two
+    // halves are two channels apart: CS = 16 (mod 32)
+    constexpr int CS0 = RH * TXP;
+    constexpr int CS = IL ? CS0 + (((16 - CS0 % 32) % 32) + 32) % 32
+                          : CS0 + (((TX % 64 - CS0 % 64) % 64) + 64) % 64;
     __shared__ __align__(16) float sX[CH * CS];
     const int tid = threadIdx.x;
     const uint8_t* img8 = (const uint8_t*)img;
@@ -598,6 +607,29 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 const int e = tid + k * NT;
                 if (NYI % NT != 0 && e >= NYI) break;
                 const int cx = e % TX, t = e / TX;
+                if constexpr (IL) {
+                    const int pr = t & 1, r0 = (t >> 1) * R;
+                    float ca[R + P - 1], cb[R + P - 1];
+#pragma unroll
+                    for (int j = 0; j < R + P - 1; ++j) {
+                        ca[j] = sX[(2 * pr) * CS + (r0 + j) * TXP + cx];
+                        cb[j] = sX[(2 * pr + 1) * CS + (r0 + j) * TXP + cx];
+                    }
+                    const int gx = x0 + cx;
+                    float2* dst = reinterpret_cast<float2*>(xy) + 2 * (off + (int64_t)z * plane + gx) + pr;
+#pragma unroll
+                    for (int o = 0; o < R; ++o) {
+                        float a = ca[o] * taps.g[0], b = cb[o] * taps.g[0];
+#pragma unroll
+                        for (int kk = 1; kk < P; ++kk) {
+                            a = fmaf(ca[o + kk], taps.g[kk], a);
+                            b = fmaf(cb[o + kk], taps.g[kk], b);
+                        }
+                        const int gy = y0 + r0 + o;
+                        if (gx < w - M && gy < h - M) dst[2 * (int64_t)gy * w] = make_float2(a, b);
+                    }
+                    continue;
+                }
                 const int ch = ONLY ? 0 : t % CH, r0 = (ONLY ? t : t / CH) * R;
                 float col[R + P - 1];
 #pragma unroll
@@ -814,7 +846,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 // compiled in keeps the plane loop one basic block).  All accesses are buffer
 // operations: the column's offset in a VGPR, the plane's offset in an SGPR
 // (blocks stay below 2^32 bytes per buffer).
-template <int P, bool LIT, int TXC, int TYC, int D, bool U8, int MODE>
+// IL (literal): the intermediate interleaved per voxel (xy[4v .. 4v+3]), one
+// 16-byte load per plane; its buffer resource is rebased at the first plane of
+// every group of P planes, so the 32-bit offsets span P + D planes (the host
+// keeps 16 (P + D) w h below 2^32).
+template <int P, bool LIT, int TXC, int TYC, int D, bool U8, int MODE, bool IL = false>
 __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) void sweep1_z_kernel(
     const float* __restrict__ xy, int64_t stride, const void* __restrict__ img,
     const float* __restrict__ blurI, float* __restrict__ out0, float* __restrict__ out1,
@@ -825,6 +861,7 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
     constexpr int M = (P - 1) / 2;
     constexpr int ESZ = U8 ? 1 : 4;
     static_assert(P % D == 0, "P must be a multiple of the load queue depth");
+    static_assert(!IL || (LIT && MODE != 4), "the interleaved layout is literal");
     (void)same_lambda;   // the host picks MODE 0 or 3 from it
     for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
         const Tile tl = tiles[item];
@@ -859,6 +896,17 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         auto st = [&](__amdgpu_buffer_rsrc_t r, int z, float v) {
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, vc, so4(z), 0);
         };
+        // interleaved layout: the four channels of plane (base + dz), r based at plane base
+        auto ld4 = [&](float (&q)[CHL], __amdgpu_buffer_rsrc_t r, int dz) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(16u * (uint32_t)c),
+                                                                 (int)(16u * (uint32_t)dz * (uint32_t)plane), 0);
+            // each element through uint32_t (a bit_cast of the vector element itself
+            // compiles to a one-dword load copied to every element)
+            const uint32_t u[4] = {v[0], v[1], v[2], v[3]};
+#pragma unroll
+            for (int ch = 0; ch < CHL; ++ch) q[ch] = __builtin_bit_cast(float, u[ch]);
+        };
+        auto ilbase = [&](int z) { return rsrc(xy + 4 * (bd.off + (int64_t)z * plane)); };
         // ring outputs d[ch] (valid) or the pad band; bI: image blur (fast mode)
         auto store_out = [&](int z, float Iv, bool valid, float bI, const float (&d)[CH]) {
             if constexpr (MODE == 4) {
@@ -913,6 +961,8 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
             for (int k = 0; k < D; ++k) qi[k] = qB[k] = 0.f;
         auto group = [&](int t0, auto guard_) __attribute__((always_inline)) {
             constexpr bool GUARD = decltype(guard_)::value;
+            __amdgpu_buffer_rsrc_t rG;
+            if constexpr (IL) rG = ilbase(t0);
 #pragma unroll
             for (int s = 0; s < P; ++s) {
                 const int t = t0 + s;
@@ -924,8 +974,12 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
                 const float vi = qi[k], vB = qB[k];
                 const int tn = t + D;
                 if (!GUARD || tn < l) {
+                    if constexpr (IL) {
+                        ld4(qc[k], rG, s + D);
+                    } else {
 #pragma unroll
-                    for (int ch = 0; ch < CHL; ++ch) qc[k][ch] = ldf(rC[ch], tn);
+                        for (int ch = 0; ch < CHL; ++ch) qc[k][ch] = ldf(rC[ch], tn);
+                    }
                     if (!GUARD || tn >= P - 1) {
                         // blur mode (4) reads neither the image nor an image blur
                         if constexpr (MODE != 4) qi[k] = ldi(tn - (P - 1) + M);
@@ -954,8 +1008,12 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (k < l) {
+                if constexpr (IL) {
+                    ld4(qc[k], ilbase(0), k);
+                } else {
 #pragma unroll
-                for (int ch = 0; ch < CHL; ++ch) qc[k][ch] = ldf(rC[ch], k);
+                    for (int ch = 0; ch < CHL; ++ch) qc[k][ch] = ldf(rC[ch], k);
+                }
             }
         }
         group(0, std::true_type{});
@@ -2019,31 +2077,35 @@ template <int P, int TX, int NT, int WPE, int TXC, int TYC, int D>
 Sweep1Variant s1m(const char* name);
 
 // split sweep 1: xy kernel with TX x TY tiles, z kernel with TXC x TYC columns
-template <int P, bool LIT, int TX, int TY, int NT, int TXC, int TYC, int D>
+template <int P, bool LIT, int TX, int TY, int NT, int TXC, int TYC, int D, bool IL = false>
 Sweep1Variant s1split(const void* xy_fn, const char* name) {
     Sweep1Variant v{P, LIT ? 1 : 0, TX, TY, NT, xy_fn, name,
-                    (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0>, TXC, TYC, {}};
+                    (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0, IL>, TXC, TYC, {}};
     v.nt2 = TXC * TYC;
-    v.fn2s[0] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 0>;
-    v.fn2s[1] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0>;
-    v.fn2s[2] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 1>;
-    v.fn2s[3] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 1>;
-    v.fn2s[4] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 2>;
-    v.fn2s[5] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 2>;
-    v.fn2s[6] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 3>;
-    v.fn2s[7] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 3>;
+    v.fn2s[0] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 0, IL>;
+    v.fn2s[1] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 0, IL>;
+    v.fn2s[2] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 1, IL>;
+    v.fn2s[3] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 1, IL>;
+    v.fn2s[4] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 2, IL>;
+    v.fn2s[5] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 2, IL>;
+    v.fn2s[6] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 3, IL>;
+    v.fn2s[7] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 3, IL>;
     v.fn2s[8] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, false, 4>;
     v.fn2s[9] = (const void*)&sweep1_z_kernel<P, LIT, TXC, TYC, D, true, 4>;
     return v;
 }
 
 template <int P, bool LIT, int TX, int TY, int NT, int R, int WPE, int TXC, int TYC, int D, int HI = 0,
-          int RX = 4, int UX = 1, int UY = 1>
+          int RX = 4, int UX = 1, int UY = 1, bool IL = false>
 Sweep1Variant s1s(const char* name) {
-    Sweep1Variant v = s1split<P, LIT, TX, TY, NT, TXC, TYC, D>(
-        (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1>, name);
-    v.fn_img[0] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 0>;
-    v.fn_img[1] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1>;
+    Sweep1Variant v = s1split<P, LIT, TX, TY, NT, TXC, TYC, D, IL>(
+        (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1, false, IL>, name);
+    // a float image's four-channel kernel on the interleaved layout spills (its
+    // prefetch holds a float4 per four voxels): it forms channel pairs instead
+    constexpr int HIF = IL && HI == 2 ? 1 : HI;
+    v.fn_img[0] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HIF, RX, UX, UY, 0, false, IL>;
+    v.fn_img[1] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1, false, IL>;
+    if constexpr (IL) v.max_plane = (int64_t)(0xffffffffu / (16u * (P + D)));
     if constexpr (!LIT) {
         v.fn_blur[0] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 0, true>;
         v.fn_blur[1] = (const void*)&sweep1_xy_kernel<P, LIT, TX, TY, NT, R, WPE, HI, RX, UX, UY, 1, true>;
@@ -2069,8 +2131,16 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         // x and y passes on the matrix cores: a tuner candidate, 1.56 ms per sweep 1 at
         // 512^3 against 1.36 for the default (MFMA at about a third of its rate)
         s1m<21, 64, 256, 2, 256, 1, 3>("p21_fast_mfma_64x28"),
-        // literal default: 32x60 tiles, 15 outputs per y item (the xy kernel specialised
-        // for the image dtype: 2.36 ms at 512^3, 15.0 at 1024x1024x768)
+        // literal default: 32x60 tiles, 15 outputs per y item, each x-pass item forming
+        // all four channels from one H/I window, the intermediate interleaved per voxel
+        // (sweep 1 at 512^3: 2.05 ms; 2.20 with channel planes, 2.24 with channel pairs
+        // per x-pass item)
+        s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, 2, 4, 1, 1, true>("p21_lit_il_32x60r15"),
+        // the same on channel planes: the default where a plane is too large for the
+        // interleaved z kernel's 32-bit offsets
+        s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, 2>("p21_lit_q4_32x60r15"),
+        // 32x60 tiles, 15 outputs per y item, one channel pair per x-pass item (the xy
+        // kernel specialised for the image dtype: 2.36 ms at 512^3, 15.0 at 1024x1024x768)
         s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, true>("p21_lit_hi_32x60r15"),
         s1s<21, true, 32, 60, 256, 10, 2, 256, 1, 3, true>("p21_lit_hi_32x60r10"),
         s1s<21, true, 32, 60, 256, 30, 2, 256, 1, 3, true>("p21_lit_hi_32x60r30"),
@@ -2078,10 +2148,9 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, true, 16, 108, 256, 9, 2, 256, 1, 3, true, 8>("p21_lit_hi_16x108x8"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true, 8>("p21_lit_hi_32x44x8"),
-        // all four channels per x-pass item from one H/I window (2.22 ms at 512^3
-        // against 2.24 for the default: within the run-to-run spread)
-        s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, 2>("p21_lit_q4_32x60r15"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, 2, 8>("p21_lit_q4_32x44x8"),
+        // more tiles on the interleaved intermediate
+        s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, 1, 4, 1, 1, true>("p21_lit_ilhi_32x60r15"),
         s1s<21, true, 32, 28, 256, 7, 4, 256, 1, 3>("p21_lit_split_32x28w4"),
 
         s1s<21, true, 64, 28, 512, 7, 4, 256, 1, 3>("p21_lit_split_64x28w4"),
