@@ -367,7 +367,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     static_assert(!ONLY || !LIT, "the image-only blur is a fast-mode pass");
     static_assert(!IL || (LIT && TX == 32 && !ONLY), "the interleaved layout: literal, 32-column tiles");
     constexpr int CH = LIT ? 4 : 2;
-    constexpr int CIN = HI ? 2 : CH;       // LDS input arrays
+    constexpr int CIN = ONLY ? 1 : (HI ? 2 : CH);   // LDS input arrays (the image-only blur: one)
     constexpr int M = (P - 1) / 2;
     constexpr int RH = TY + 2 * M, RW = TX + 2 * M;
     constexpr int RWP = (RW + 3) & ~3;
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     constexpr int CS0 = RH * TXP;
     constexpr int CS = IL ? CS0 + (((16 - CS0 % 32) % 32) + 32) % 32
                           : CS0 + (((TX % 64 - CS0 % 64) % 64) + 64) % 64;
-    __shared__ __align__(16) float sX[CH * CS];
+    __shared__ __align__(16) float sX[(ONLY ? 1 : CH) * CS];
     const int tid = threadIdx.x;
     const uint8_t* img8 = (const uint8_t*)img;
     const float* imgf = (const float*)img;
