@@ -1644,17 +1644,21 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                 norm_comp3<LIT, decltype(EX_)::value>(gx[i], gy[i], gz[i], nx[i], ny[i], nz[i], bad);
                 if constexpr (LIT) __builtin_amdgcn_sched_barrier(0);   // one point's temporaries at a time
             }
-            if (share) {
+            if (share && active) {
                 sNY[b][ty + 1][tx] = make_float4(ny[0], ny[1], ny[2], ny[3]);
                 sGY[b][ty + 1][tx] = make_float4(gy[0], gy[1], gy[2], gy[3]);
                 sL[b][ty][tx + 1] = make_float2(nx[3], gx[3]);
                 sR[b][ty][tx + 1] = make_float2(nx[0], gx[0]);
             }
         };
-        const uint32_t v0 = 4u * (uint32_t)(zb * P + y * W + x0);
+        // Every lane loads and computes (no divergent branch around the register
+        // rings, which would make the compiler copy them): an inactive lane reads
+        // the item's first voxel instead of its own, and only its stores, LDS
+        // writes and flag are masked.
+        const uint32_t v0 = 4u * (uint32_t)(zb * P + (active ? y : tl.y0) * W + (active ? x0 : tl.x0));
         bool bad = false;   // literal, FIX = 0: some point this lane computed is outside the fast forms' range
         // ---- prologue: rows of planes zb-2 .. zb+2, own points at zb-1 and zb, halo at zb
-        if (active) {
+        {
             float Cmm[6], Um[4], Dm[4];
             load6(Cmm, v0, S(0, -2));
             load6(C[3], v0, S(0, -1));
@@ -1682,14 +1686,14 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
             constexpr int s = decltype(S_)::value;
             constexpr int km1 = (s + 3) & 3, k0 = s & 3, kp1 = (s + 1) & 3, kp2 = (s + 2) & 3;
             constexpr int q = s & 1, qn = (s + 1) & 1;
-            if (active && z + 1 < zend) {
-                load6(C[km1], v + vstep, S(0, 2));      // plane z+3 (slot of z-1)
-                load4(U[q], v + vstep, S(1, 1));        // plane z+2 (slot of z)
-                load4(D[q], v + vstep, S(-1, 1));
-                loadF(FF[qn], v + vstep);               // plane z+1
-            }
+            // plane z+3 (slot of z-1), or z+2 at the chunk's last plane, whose
+            // loads feed no later step (z+3 may lie past the block)
+            load6(C[km1], v + vstep, S(0, z + 1 < zend ? 2 : 1));
+            load4(U[q], v + vstep, S(1, 1));            // plane z+2 (slot of z)
+            load4(D[q], v + vstep, S(-1, 1));
+            loadF(FF[qn], v + vstep);                   // plane z+1
             constexpr bool EX = FIX != 0;
-            if (active) {
+            {
                 // own points at plane z+1: the z neighbours now, shared for the next plane
                 own(std::integral_constant<bool, EX>{}, bad, C[k0], C[kp1], C[kp2], U[qn], D[qn], NX[qn], GX[qn],
                     NZ[kp1], GZ[kp1], qn, z + 1 < zend);
@@ -1714,8 +1718,10 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                     pt[i] = __builtin_bit_cast(uint32_t, t1);
                     if constexpr (LIT) __builtin_amdgcn_sched_barrier(0);
                 }
-                __builtin_amdgcn_raw_buffer_store_b128(po, rO, (int)v, 0, 0);
-                __builtin_amdgcn_raw_buffer_store_b128(pt, rT, (int)v, 0, 0);
+                if (active) {
+                    __builtin_amdgcn_raw_buffer_store_b128(po, rO, (int)v, 0, 0);
+                    __builtin_amdgcn_raw_buffer_store_b128(pt, rT, (int)v, 0, 0);
+                }
             }
             if (z + 1 < zend) halo(std::integral_constant<bool, EX>{}, bad, z + 1, qn);
             // this plane's reads of buffer q precede the next plane's writes to it
@@ -2199,6 +2205,8 @@ const std::vector<Sweep2Variant>& sweep2_variants() {
         s2q<false, 16, 16, 2>("fast_vec_16x16"),
         s2q<false, 16, 8, 2>("fast_vec_16x8"),
         s2h<false, 32, 8, 2>("fast_share_32x8"),
+        s2h<false, 32, 8, 4>("fast_share_32x8w4"),
+        s2h<false, 16, 16, 4>("fast_share_16x16w4"),
         s2s<false, 64, 4>("fast_split_64x4"),
         s2s<false, 64, 2>("fast_split_64x2"),
         s2s<false, 32, 8>("fast_split_32x8"),
@@ -2209,6 +2217,8 @@ const std::vector<Sweep2Variant>& sweep2_variants() {
         s2h<true, 32, 16, 2>("lit_share_32x16"),
         s2q<true, 32, 8, 2>("lit_vec_32x8"),
         s2h<true, 32, 8, 2>("lit_share_32x8"),
+        s2h<true, 32, 8, 3>("lit_share_32x8w3"),
+        s2h<true, 16, 16, 3>("lit_share_16x16w3"),
         s2h<true, 16, 16, 2>("lit_share_16x16"),
         s2s<true, 64, 4>("lit_split_64x4"),
         s2v<true, 64, 4>("lit_64x4"),
