@@ -323,6 +323,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WP
 // in the same order as the fused kernel: identical results.  Channels: fast
 // mode I*G and G; literal mode I*H, H, I*(1-H), 1-H (main.cpp:238-246).
 
+// 16- and 8-byte buffer-load results
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 // Channel values of one voxel from the transform t = T(phi) and the image i.
 template <bool LIT>
 __device__ __forceinline__ void sweep1_channels(float i, float t, float (&c)[LIT ? 4 : 2]) {
@@ -412,8 +416,45 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         const bool vec = ((off | (int64_t)w) & 3) == 0 && ((uintptr_t)tr & 15) == 0 &&
                          (img_u8 ? ((uintptr_t)img8 & 3) == 0 : ((uintptr_t)imgf & 15) == 0);
         float4 pt[PF], pi[PF];
+        // 16-byte aligned blocks: each prefetch round's element group as an index
+        // inside a plane (the window's rows clamped to the block, its last group
+        // to w-4: columns past w-1 only feed outputs that are not stored; the last
+        // round's spare threads repeat the last group, and store() skips them),
+        // loaded through buffer resources based at the plane
+        int eidx[PF];
+        if (vec) {
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const int e = min(tid + k * NT, NQ - 1);
+                const int r = e / QW, q = e - r * QW;
+                eidx[k] = min(y0 - M + r, h - 1) * w + min(x0 - M + 4 * q, w - 4);
+            }
+        }
         auto fetch = [&](int z) {
             const int64_t zoff = off + (int64_t)z * plane;
+            if (vec) {
+                const __amdgpu_buffer_rsrc_t rT =
+                    __builtin_amdgcn_make_buffer_rsrc((void*)(tr + zoff), 0, (int)0xffffffff, 0x00020000);
+                const __amdgpu_buffer_rsrc_t rI = __builtin_amdgcn_make_buffer_rsrc(
+                    img_u8 ? (void*)(img8 + zoff) : (void*)(imgf + zoff), 0, (int)0xffffffff, 0x00020000);
+#pragma unroll
+                for (int k = 0; k < PF; ++k) {
+                    const uint32_t vo = 4u * (uint32_t)eidx[k];
+                    if constexpr (!ONLY) {
+                        const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rT, (int)vo, 0, 0);
+                        pt[k] = make_float4(__uint_as_float(b[0]), __uint_as_float(b[1]), __uint_as_float(b[2]),
+                                            __uint_as_float(b[3]));
+                    }
+                    if (img_u8) {
+                        pi[k].x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rI, (int)(vo >> 2), 0, 0));
+                    } else {
+                        const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rI, (int)vo, 0, 0);
+                        pi[k] = make_float4(__uint_as_float(b[0]), __uint_as_float(b[1]), __uint_as_float(b[2]),
+                                            __uint_as_float(b[3]));
+                    }
+                }
+                return;
+            }
 #pragma unroll
             for (int k = 0; k < PF; ++k) {
                 const int e = tid + k * NT;
@@ -484,10 +525,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             __syncthreads();
             // ---- x pass (one item at a time: registers, not latency, are the limit)
             if constexpr (HI == 2) {
+                // items that would fill only part of a last round (e.g. 640 items on
+                // 256 threads) run that round as channel-pair halves over all the
+                // threads, so that every wave issues the same x-pass work
+                constexpr int NF = NXI / NT, REM = NXI % NT;
+                constexpr bool SPLIT = REM > 0 && 2 * REM == NT;
 #pragma unroll UX
-                for (int k = 0; k < NXR; ++k) {
+                for (int k = 0; k < (SPLIT ? NF : NXR); ++k) {
                     const int e = tid + k * NT;
-                    if (NXI % NT != 0 && e >= NXI) break;
+                    if (!SPLIT && NXI % NT != 0 && e >= NXI) break;
                     const int r = e / XG;
                     const int xb = (e - r * XG) * RX;
                     float cw[4][WIN];
@@ -522,6 +568,51 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                             *reinterpret_cast<float4*>(&sX[ch * CS + r * TXP + xb + o]) =
                                 make_float4(oc[o], oc[o + 1], oc[o + 2], oc[o + 3]);
                     }
+                }
+                if constexpr (SPLIT) {
+                    // pair 0: (I*H, H), pair 1: (I*(1-H), 1-H), the values of sweep1_channels
+                    auto half = [&](auto PR_) __attribute__((always_inline)) {
+                        constexpr int pr = decltype(PR_)::value;
+                        const int e = NF * NT + tid % REM;
+                        const int r = e / XG;
+                        const int xb = (e - r * XG) * RX;
+                        float wa[WIN], wb[WIN];
+                        const float4* sh = reinterpret_cast<const float4*>(&sIn[0][r][xb]);
+                        const float4* si = reinterpret_cast<const float4*>(&sIn[1][r][xb]);
+#pragma unroll
+                        for (int q = 0; q < WIN / 4; ++q) {
+                            const float4 h4 = sh[q], i4 = si[q];
+                            const float hv[4] = {h4.x, h4.y, h4.z, h4.w}, iv[4] = {i4.x, i4.y, i4.z, i4.w};
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const float t = pr == 0 ? hv[j] : 1.0f - hv[j];
+                                wa[4 * q + j] = iv[j] * t;
+                                wb[4 * q + j] = t;
+                            }
+                        }
+                        float oa[RX], ob[RX];
+#pragma unroll
+                        for (int o = 0; o < RX; ++o) {
+                            float a = wa[o] * taps.g[0], b = wb[o] * taps.g[0];
+#pragma unroll
+                            for (int kk = 1; kk < P; ++kk) {
+                                a = fmaf(wa[o + kk], taps.g[kk], a);
+                                b = fmaf(wb[o + kk], taps.g[kk], b);
+                            }
+                            oa[o] = a;
+                            ob[o] = b;
+                        }
+#pragma unroll
+                        for (int o = 0; o < RX; o += 4) {
+                            *reinterpret_cast<float4*>(&sX[(2 * pr) * CS + r * TXP + xb + o]) =
+                                make_float4(oa[o], oa[o + 1], oa[o + 2], oa[o + 3]);
+                            *reinterpret_cast<float4*>(&sX[(2 * pr + 1) * CS + r * TXP + xb + o]) =
+                                make_float4(ob[o], ob[o + 1], ob[o + 2], ob[o + 3]);
+                        }
+                    };
+                    // REM is a multiple of the wave size: the choice is wave-uniform
+                    if (tid < REM) half(std::integral_constant<int, 0>{});
+                    else half(std::integral_constant<int, 1>{});
                 }
             } else if constexpr (HI) {
 #pragma unroll UX
@@ -608,7 +699,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 if (NYI % NT != 0 && e >= NYI) break;
                 const int cx = e % TX, t = e / TX;
                 if constexpr (IL) {
-                    const int pr = t & 1, r0 = (t >> 1) * R;
+                    // a wave's two halves take the two pairs of the same 32 columns
+                    // and rows: the row group is wave-uniform
+                    const int pr = t & 1, r0 = __builtin_amdgcn_readfirstlane((t >> 1) * R);
                     float ca[R + P - 1], cb[R + P - 1];
 #pragma unroll
                     for (int j = 0; j < R + P - 1; ++j) {
@@ -616,7 +709,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                         cb[j] = sX[(2 * pr + 1) * CS + (r0 + j) * TXP + cx];
                     }
                     const int gx = x0 + cx;
-                    float2* dst = reinterpret_cast<float2*>(xy) + 2 * (off + (int64_t)z * plane + gx) + pr;
+                    const bool xin = gx < w - M;
+                    // the plane's interleaved channels through a buffer resource: the
+                    // column offset per lane, the row offset in an SGPR (16 w h < 2^32
+                    // on this layout)
+                    const __amdgpu_buffer_rsrc_t rX = __builtin_amdgcn_make_buffer_rsrc(
+                        (void*)(xy + 4 * (off + (int64_t)z * plane)), 0, (int)0xffffffff, 0x00020000);
+                    const int vo = 16 * gx + 8 * pr;
 #pragma unroll
                     for (int o = 0; o < R; ++o) {
                         float a = ca[o] * taps.g[0], b = cb[o] * taps.g[0];
@@ -626,7 +725,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                             b = fmaf(cb[o + kk], taps.g[kk], b);
                         }
                         const int gy = y0 + r0 + o;
-                        if (gx < w - M && gy < h - M) dst[2 * (int64_t)gy * w] = make_float2(a, b);
+                        if (gy < h - M && xin) {
+                            const u32x2 v2 = {__float_as_uint(a), __float_as_uint(b)};
+                            __builtin_amdgcn_raw_buffer_store_b64(v2, rX, vo, (int)(16u * (uint32_t)gy * (uint32_t)w), 0);
+                        }
                     }
                     continue;
                 }
@@ -1242,8 +1344,6 @@ __global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
 // Per step a thread loads one new C row, one new U and D row, the rows
 // y+-2 and F (12 loads for 4 voxels) and evaluates 14 gradients (3.5 per
 // voxel).  Interior rules only: x in [4, 4+4*nq), y and z in [2, n-2).
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 // Normal components at one point.  Literal path: EX selects the exact forms;
 // otherwise the fast forms run and `bad` records whether this lane's inputs
