@@ -132,6 +132,7 @@ struct Sweep2Variant {
     const void* scalar_fn; // vector variants: interior kernel of blocks with unaligned rows
     const char* name;
     const void* fix_fn;    // literal vector variant: exact recompute of the items the interior flagged
+    int fullx = 0;         // the interior kernel covers whole rows (x in [0, w)) of aligned blocks
 };
 const std::vector<Sweep1Variant>& sweep1_variants();
 const std::vector<Sweep2Variant>& sweep2_variants();

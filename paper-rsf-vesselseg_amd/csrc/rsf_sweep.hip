@@ -1181,7 +1181,8 @@ __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const 
 
 // Sweep 2, boundary shell: voxels within 2 of a face of their block, and,
 // for blocks whose interior runs on the vector kernel, the columns that kernel
-// leaves out (x = 2, 3 and x >= 4 + 4*((w-8)/4)).  Work items (block, first
+// leaves out (x = 2, 3 and x >= 4 + 4*((w-8)/4)); the shared-interior kernel
+// covers whole rows, and its shell is the planes and rows only.  Work items (block, first
 // shell index, count, vector flag) enumerate, per block, the 4 planes z in
 // {0,1,l-2,l-1}, then the rows y in {0,1,h-2,h-1} of the inner planes, then the
 // boundary columns of the inner rows.
@@ -1195,8 +1196,10 @@ __global__ __launch_bounds__(256) void sweep2_shell_kernel(
         if (t.b < 0 || (int)threadIdx.x >= t.y0) continue;
         const BlockDesc bd = blocks[t.b];
         const int w = bd.w, h = bd.h, l = bd.l;
-        // boundary columns of the inner rows: x < cl and x >= w - cr
-        const int cl = t.z0 ? 4 : 2, cr = t.z0 ? w - 4 - 4 * ((w - 8) >> 2) : 2, cols = cl + cr;
+        // boundary columns of the inner rows: x < cl and x >= w - cr (z0: 0 scalar
+        // interior, 1 vector interior, 2 an interior over whole rows: no columns)
+        const int cl = t.z0 == 2 ? 0 : (t.z0 ? 4 : 2);
+        const int cr = t.z0 == 2 ? 0 : (t.z0 ? w - 4 - 4 * ((w - 8) >> 2) : 2), cols = cl + cr;
         int64_t k = (int64_t)t.x0 + threadIdx.x;
         const int64_t cA = 4 * (int64_t)w * h, cB = 4 * (int64_t)w * (l - 4);
         int x, y, z;
@@ -1619,8 +1622,10 @@ __device__ __forceinline__ void norm_comp3(float gx, float gy, float gz, float& 
 // one, x0-1 of the first quad, the column after the last active quad) have no
 // owner: all threads of the workgroup compute those points together.  About
 // 1.3 normal evaluations per voxel instead of 3.5, and rings of 4-wide rows.
-// The same operations on the same values as the other sweep-2 kernels: the
-// same bits.  FIX as in the vector kernel, except that a flagged item is
+// It covers whole rows, x in [0, w) (y and z in [2, n-2)), of blocks whose rows
+// start on 16-byte boundaries, so its boundary shell is the planes and rows
+// within 2 of a face only.  The same operations on the same values as the
+// other sweep-2 kernels: the same bits.  FIX as in the vector kernel, except that a flagged item is
 // recomputed whole (its lanes read normals that other waves computed).
 template <bool LIT, int TX, int TY, int WPE, int FIX = 0>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep2_share_kernel(
@@ -1645,12 +1650,15 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
         }
         const BlockDesc bd = blocks[tl.b];
         const int w = bd.w, h = bd.h, l = bd.l;
-        const int xend = 4 + 4 * ((w - 8) >> 2);
-        const int xe = min(tl.x0 + 4 * TX, xend);    // end of the item's active columns
+        // whole rows: the quad at x = 0 reads its left neighbour as the previous
+        // linear element, (w-1, y-1) -- the reference's x-wrap -- and the quad
+        // ending at w-1 takes the one-sided rule there (kernel.cu:64-74)
+        const int xe = min(tl.x0 + 4 * TX, w);        // end of the item's active columns
         const int yb = min(tl.y0 + TY, h - 2);        // end of its active rows
         const int nq = (xe - tl.x0) >> 2;             // active quads per row
         const int x0 = tl.x0 + 4 * tx, y = tl.y0 + ty;
         const bool active = tx < nq && y < yb;
+        const bool qlast = x0 + 4 == w;               // the quad ends at x = w-1
         const int zb = tl.z0, zend = min(tl.z0 + zc, l - 2);
         const int W = w, P = w * h;
         const int shift = 2 * P + 2 * W + 4;
@@ -1704,13 +1712,16 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                     idx = u - side * TY;
                     yh = tl.y0 + idx;
                     x = side ? xe : tl.x0 - 1;
-                    if (yh >= yb) continue;
+                    // right of x = w-1 the reference takes the point itself (see the step)
+                    if (yh >= yb || x >= w) continue;
                 }
                 const uint32_t vp = 4u * (uint32_t)(zp * P + yh * W + x) + 16u;
                 auto ld = [&](int dx, int dy, int dz) {
                     return u2f(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(vp + 4 * dx), S(dy, dz), 0));
                 };
-                const float gx = (ld(1, 0, 0) - ld(-1, 0, 0)) * 0.5f;
+                // x = w-1, and x = -1 (the previous row's last voxel), difference one-sided
+                const bool xl = x == w - 1 || x == -1;
+                const float gx = ((xl ? ld(0, 0, 0) : ld(1, 0, 0)) - ld(-1, 0, 0)) * 0.5f;
                 const float gy = (ld(0, 1, 0) - ld(0, -1, 0)) * 0.5f;
                 const float gz = (ld(0, 0, 1) - ld(0, 0, -1)) * 0.5f;
                 if (row) {
@@ -1738,7 +1749,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
             float ny[4], gy[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                gx[i] = (c0[i + 2] - c0[i]) * 0.5f;
+                gx[i] = ((i == 3 && qlast ? c0[i + 1] : c0[i + 2]) - c0[i]) * 0.5f;
                 gy[i] = (u[i] - d[i]) * 0.5f;
                 gz[i] = (cp[i + 1] - cm[i + 1]) * 0.5f;
                 norm_comp3<LIT, decltype(EX_)::value>(gx[i], gy[i], gz[i], nx[i], ny[i], nz[i], bad);
@@ -1803,8 +1814,9 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                 const float2 L = sL[q][ty][tx], R = sR[q][ty][tx + 2];
                 const float nyus[4] = {nyu.x, nyu.y, nyu.z, nyu.w}, nyds[4] = {nyd.x, nyd.y, nyd.z, nyd.w};
                 const float gyus[4] = {gyu.x, gyu.y, gyu.z, gyu.w}, gyds[4] = {gyd.x, gyd.y, gyd.z, gyd.w};
-                const float nxs[6] = {L.x, NX[q][0], NX[q][1], NX[q][2], NX[q][3], R.x};
-                const float gxs[6] = {L.y, GX[q][0], GX[q][1], GX[q][2], GX[q][3], R.y};
+                // at x = w-1 the right neighbour is the point itself
+                const float nxs[6] = {L.x, NX[q][0], NX[q][1], NX[q][2], NX[q][3], qlast ? NX[q][3] : R.x};
+                const float gxs[6] = {L.y, GX[q][0], GX[q][1], GX[q][2], GX[q][3], qlast ? GX[q][3] : R.y};
                 u32x4 po, pt;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -2175,7 +2187,7 @@ template <bool LIT, int TX, int TY, int WPE>
 Sweep2Variant s2h(const char* name) {
     return Sweep2Variant{LIT ? 1 : 0, TX, TY, 4, (const void*)&sweep2_share_kernel<LIT, TX, TY, WPE, 0>,
                          (const void*)&sweep2_shell_kernel<LIT>, (const void*)&sweep2_inner_kernel<LIT, 64, 4>,
-                         name, LIT ? (const void*)&sweep2_share_kernel<LIT, TX, TY, WPE, 1> : nullptr};
+                         name, LIT ? (const void*)&sweep2_share_kernel<LIT, TX, TY, WPE, 1> : nullptr, 1};
 }
 
 // split sweep 1, fast mode, xy kernel on the matrix cores (TX x 28 tiles)
@@ -2312,14 +2324,17 @@ const std::vector<Sweep2Variant>& sweep2_variants() {
         s2s<false, 32, 8>("fast_split_32x8"),
         s2v<false, 64, 4>("fast_64x4"),
         s2l<false, 32, 16, 4>("fast_lds_32x16"),
-        // literal default (the first literal entry): 1.02 ms per sweep 2 at 512^3 against
-        // 1.12 for the vector kernel, whose normals are evaluated 3.5 times per voxel
+        // literal default (the first literal entry): the shared interior over whole rows
+        // on 16 x 16 quads, three 256-thread workgroups per CU (159 VGPRs): 0.76 ms per
+        // sweep 2 at 512^3 with its shell, against 0.81 for 32 x 16 quads (one 512-thread
+        // workgroup per CU) and 1.12 for the vector kernel, whose normals are evaluated
+        // 3.5 times per voxel
+        s2h<true, 16, 16, 2>("lit_share_16x16"),
         s2h<true, 32, 16, 2>("lit_share_32x16"),
         s2q<true, 32, 8, 2>("lit_vec_32x8"),
         s2h<true, 32, 8, 2>("lit_share_32x8"),
         s2h<true, 32, 8, 3>("lit_share_32x8w3"),
         s2h<true, 16, 16, 3>("lit_share_16x16w3"),
-        s2h<true, 16, 16, 2>("lit_share_16x16"),
         s2s<true, 64, 4>("lit_split_64x4"),
         s2v<true, 64, 4>("lit_64x4"),
     };
@@ -2395,11 +2410,13 @@ int plan_tiles_host(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, 
             const bool vec = V2[s2].vec == 4 && (bd.off & 3) == 0 && (bd.w & 3) == 0;
             int cols = 4;
             if (vec) {
-                const int xend = 4 + 4 * ((bd.w - 8) >> 2);
-                cols = 4 + (bd.w - xend);
+                // the shared-interior kernel takes whole rows, the vector kernel x in [4, xend)
+                const int xbeg = V2[s2].fullx ? 0 : 4;
+                const int xend = V2[s2].fullx ? bd.w : 4 + 4 * ((bd.w - 8) >> 2);
+                cols = V2[s2].fullx ? 0 : 4 + (bd.w - xend);
                 for (int z = 2; z < bd.l - 2; z += zc2)
                     for (int y = 2; y < bd.h - 2; y += ty2)
-                        for (int x = 4; x < xend; x += 4 * tx2) tp.tiles2.push_back(Tile{(int)b, x, y, z});
+                        for (int x = xbeg; x < xend; x += 4 * tx2) tp.tiles2.push_back(Tile{(int)b, x, y, z});
             } else {
                 std::vector<Tile>& dst = V2[s2].vec == 4 ? tp.tiles4 : tp.tiles2;
                 const int txs = V2[s2].vec == 4 ? 64 : tx2, tys = V2[s2].vec == 4 ? 4 : ty2;
@@ -2410,7 +2427,8 @@ int plan_tiles_host(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, 
             const int64_t shell = 4 * (int64_t)bd.w * bd.h + 4 * (int64_t)bd.w * (bd.l - 4) +
                                   (int64_t)cols * (bd.h - 4) * (bd.l - 4);
             for (int64_t k = 0; k < shell; k += 256)
-                tp.tiles3.push_back(Tile{(int)b, (int)k, (int)std::min<int64_t>(256, shell - k), vec ? 1 : 0});
+                tp.tiles3.push_back(
+                    Tile{(int)b, (int)k, (int)std::min<int64_t>(256, shell - k), vec ? (V2[s2].fullx ? 2 : 1) : 0});
         } else {
             for (int z = 0; z < bd.l; z += zc2)
                 for (int y = 0; y < bd.h; y += ty2)
