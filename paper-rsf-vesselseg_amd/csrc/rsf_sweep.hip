@@ -421,6 +421,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         const int64_t off = bd.off, plane = (int64_t)w * h;
         const int x0 = tl.x0, y0 = tl.y0, z0 = tl.z0;
         const int zend = min(z0 + zc, l);
+        // four-channel literal tiles past the valid region's last row (e.g. 492
+        // rows in tiles of 60) skip the window rows and y-pass rows they do not
+        // need (in the fast kernel the extra branch cost more than it saved)
+        const int ny = min(TY, h - M - y0), rxv = ny + 2 * M;
         const bool vec = ((off | (int64_t)w) & 3) == 0 && ((uintptr_t)tr & 15) == 0 &&
                          (img_u8 ? ((uintptr_t)img8 & 3) == 0 : ((uintptr_t)imgf & 15) == 0);
         float4 pt[PF], pi[PF];
@@ -543,6 +547,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                     const int e = tid + k * NT;
                     if (!SPLIT && NXI % NT != 0 && e >= NXI) break;
                     const int r = e / XG;
+                    if (r >= rxv) continue;
                     const int xb = (e - r * XG) * RX;
                     float cw[4][WIN];
                     const float4* sh = reinterpret_cast<const float4*>(&sIn[0][r][xb]);
@@ -583,6 +588,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                         constexpr int pr = decltype(PR_)::value;
                         const int e = NF * NT + tid % REM;
                         const int r = e / XG;
+                        if (r >= rxv) return;
                         const int xb = (e - r * XG) * RX;
                         float wa[WIN], wb[WIN];
                         const float4* sh = reinterpret_cast<const float4*>(&sIn[0][r][xb]);
@@ -710,6 +716,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                     // a wave's two halves take the two pairs of the same 32 columns
                     // and rows: the row group is wave-uniform
                     const int pr = t & 1, r0 = __builtin_amdgcn_readfirstlane((t >> 1) * R);
+                    if (r0 >= ny) continue;
                     float ca[R + P - 1], cb[R + P - 1];
 #pragma unroll
                     for (int j = 0; j < R + P - 1; ++j) {
