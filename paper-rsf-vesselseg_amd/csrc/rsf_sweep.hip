@@ -954,8 +954,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     }
 }
 
-// z kernel: one thread per (x, y) column of a block, TXC x TYC columns per work
-// item, marching all l planes.  Columns inside the valid xy region run the z
+// z kernel: one thread per (x, y) column of a block, TXC x TYC consecutive
+// columns per work item, marching all l planes.  Columns inside the valid xy region run the z
 // ring over the xy-blurred channels; every voxel outside the valid region is
 // in the pad band (fits 30/30 = 1).  U8: image dtype; MODE 0 writes F with
 // lambda1 == lambda2, 3 F with separate lambdas, 1 the fits, 2 the fits and
@@ -985,9 +985,12 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         if (tl.b < 0) continue;
         const BlockDesc bd = blocks[tl.b];
         const int w = bd.w, h = bd.h, l = bd.l;
-        const int x = tl.x0 + (int)threadIdx.x % TXC, y = tl.y0 + (int)threadIdx.x / TXC;
-        if (x >= w || y >= h) continue;
+        // a work item is TXC*TYC consecutive columns of the block in (y, x) order
+        // (whole waves stay busy on narrow blocks); no state crosses columns
         const int plane = w * h;
+        const int c = tl.x0 + (int)threadIdx.x;
+        if (c >= plane) continue;
+        const int y = c / w, x = c - y * w;
         auto rsrc = [](const void* q) {
             return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(q), 0, (int)0xffffffff, 0x00020000);
         };
@@ -998,8 +1001,7 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         const __amdgpu_buffer_rsrc_t rI = rsrc((const char*)img + bd.off * ESZ);
         const __amdgpu_buffer_rsrc_t r0 = rsrc(out0 + bd.off), r1 = rsrc(out1 + bd.off);
         const __amdgpu_buffer_rsrc_t r2 = rsrc(out2 + bd.off), r3 = rsrc(out3 + bd.off);
-        const int c = y * w + x;                 // element offset of the column
-        const int vc = 4 * c, vci = ESZ * c;     // byte offsets (VGPR)
+        const int vc = 4 * c, vci = ESZ * c;     // byte offsets of the column (VGPR)
         // plane byte offsets (SGPR) in unsigned 32-bit arithmetic: a block spans
         // fewer than 2^32 bytes per buffer, but 4 * z * plane can exceed 2^31
         auto so4 = [&](int z) { return (int)(4u * (uint32_t)z * (uint32_t)plane); };
@@ -2413,8 +2415,9 @@ int plan_tiles_host(TilePlan& tp, const std::vector<BlockDesc>& blocks, int s1, 
             for (int z = 0; z < bd.l; z += zc1)
                 for (int y = M1; y < bd.h - M1; y += ty1)
                     for (int x = M1; x < bd.w - M1; x += tx1) tp.tiles1.push_back(Tile{(int)b, x, y, z});
-            for (int y = 0; y < bd.h; y += V1[s1].ty2)
-                for (int x = 0; x < bd.w; x += V1[s1].tx2) tp.tiles5.push_back(Tile{(int)b, x, y, 0});
+            const int64_t cols = (int64_t)bd.w * bd.h;
+            for (int64_t c = 0; c < cols; c += V1[s1].tx2 * V1[s1].ty2)
+                tp.tiles5.push_back(Tile{(int)b, (int)c, 0, 0});
         } else {
             for (int z = 0; z < bd.l; z += zc1)
                 for (int y = 0; y < bd.h; y += ty1)
