@@ -1150,12 +1150,33 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma clang fp contract(off)
 
 // One voxel of sweep 2 with the full reference edge rules (boundary shell).
+// The shell runs beside interior kernels that address a block through 32-bit
+// buffer offsets, so its blocks hold fewer than 2^30 voxels: the indices are
+// 32-bit and the loads go through buffer resources.  Literal normals take the
+// shortened forms of the interior kernels and, for a |grad|^2 outside their
+// range, the exact ones (the same bits either way).
 template <bool LIT>
 __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const float* __restrict__ F,
                                              float* __restrict__ phi_out, float* __restrict__ tr_out, int x,
                                              int y, int z, int w, int h, int l, float dt, float nu, float mu,
                                              float eps, float inv_eps) {
-    const int64_t i = ((int64_t)z * h + y) * w + x;
+    const int P = w * h;
+    const int i = (z * h + y) * w + x;
+    const __amdgpu_buffer_rsrc_t rf = __builtin_amdgcn_make_buffer_rsrc((void*)f, 0, (int)0xffffffff, 0x00020000);
+    auto ld = [&](int k) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rf, (int)(4u * (uint32_t)k), 0, 0));
+    };
+    // D f at point q (linear index qi) with the edge rules of kernel.cu:57-125
+    auto grad = [&](int qx, int qy, int qz, int qi) {
+        const int xr = qx == w - 1 ? qi : qi + 1, xl = qi == 0 ? 0 : qi - 1;
+        const int yr = qy == h - 1 ? qi : qi + w, yl = qy == 0 ? qi : qi - w;
+        const int zr = qz == l - 1 ? qi : qi + P, zl = qz == 0 ? qi : qi - P;
+        Grad g;
+        g.x = (ld(xr) - ld(xl)) * 0.5f;
+        g.y = (ld(yr) - ld(yl)) * 0.5f;
+        g.z = (ld(zr) - ld(zl)) * 0.5f;
+        return g;
+    };
     int qlx, qly, qlz;
     left_x(x, y, z, w, h, qlx, qly, qlz);
     const int rxx = (x == w - 1) ? x : x + 1;
@@ -1163,17 +1184,21 @@ __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const 
     const int lyy = (y == 0) ? 0 : (y == h - 1 ? h - 2 : y - 1);
     const int rzz = (z == 0) ? 1 : (z == l - 1 ? z : z + 1);
     const int lzz = (z == 0) ? 0 : (z == l - 1 ? l - 2 : z - 1);
-    const Grad gRx = grad_at(f, rxx, y, z, w, h, l);
-    const Grad gLx = grad_at(f, qlx, qly, qlz, w, h, l);
-    const Grad gRy = grad_at(f, x, ryy, z, w, h, l);
-    const Grad gLy = grad_at(f, x, lyy, z, w, h, l);
-    const Grad gRz = grad_at(f, x, y, rzz, w, h, l);
-    const Grad gLz = grad_at(f, x, y, lzz, w, h, l);
+    const int row = i - x;
+    const Grad gRx = grad(rxx, y, z, row + rxx);
+    const Grad gLx = grad(qlx, qly, qlz, (qlz * h + qly) * w + qlx);
+    const Grad gRy = grad(x, ryy, z, i + (ryy - y) * w);
+    const Grad gLy = grad(x, lyy, z, i + (lyy - y) * w);
+    const Grad gRz = grad(x, y, rzz, i + (rzz - z) * P);
+    const Grad gLz = grad(x, y, lzz, i + (lzz - z) * P);
     float nRx, nLx, nRy, nLy, nRz, nLz;
     if constexpr (LIT) {
         auto nrm = [](const Grad& g, float c) {
-            const float M = sqrtf(mag2<LIT>(g.x, g.y, g.z));
-            return (float)((double)c / ((double)M + 1e-6));
+            const float m2 = mag2<LIT>(g.x, g.y, g.z);
+            float n[1];
+            if (normal_outside(m2)) normal_literal_exact<1>({c}, m2, n);
+            else normal_literal_fast<1>({c}, m2, n);
+            return n[0];
         };
         nRx = nrm(gRx, gRx.x); nLx = nrm(gLx, gLx.x);
         nRy = nrm(gRy, gRy.y); nLy = nrm(gLy, gLy.y);
@@ -1189,7 +1214,7 @@ __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const 
     }
     const float kap = half_sum3(nRx - nLx, nRy - nLy, nRz - nLz);
     const float lap = half_sum3(gRx.x - gLx.x, gRy.y - gLy.y, gRz.z - gLz.z);
-    const float p0 = f[i];
+    const float p0 = ld(i);
     const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
     const float p1 = phi_update<LIT>(p0, d, F[i], kap, lap, dt, nu * dt, mu);
     phi_out[i] = p1;
