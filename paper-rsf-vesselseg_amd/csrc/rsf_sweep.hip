@@ -2353,13 +2353,18 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
 
 const std::vector<Sweep2Variant>& sweep2_variants() {
     static const std::vector<Sweep2Variant> v = {
+        // fast default (the first fast entry): the shared interior over whole rows at four
+        // waves per SIMD (127 VGPRs), two 512-thread workgroups per CU: 0.51 ms per sweep 2
+        // at 512^3 with its shell, against 0.55 on 32 x 8 quads and 0.70 for the vector
+        // kernel on 64 x 4 quads
+        s2h<false, 32, 16, 4>("fast_share_32x16w4"),
+        s2h<false, 32, 8, 4>("fast_share_32x8w4"),
         s2q<false, 64, 4, 2>("fast_vec_64x4"),
         s2q<false, 32, 8, 2>("fast_vec_32x8"),
         s2q<false, 32, 4, 2>("fast_vec_32x4"),
         s2q<false, 16, 16, 2>("fast_vec_16x16"),
         s2q<false, 16, 8, 2>("fast_vec_16x8"),
         s2h<false, 32, 8, 2>("fast_share_32x8"),
-        s2h<false, 32, 8, 4>("fast_share_32x8w4"),
         s2h<false, 16, 16, 4>("fast_share_16x16w4"),
         s2s<false, 64, 4>("fast_split_64x4"),
         s2s<false, 64, 2>("fast_split_64x2"),
