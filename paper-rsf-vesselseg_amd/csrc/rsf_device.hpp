@@ -42,6 +42,15 @@ __device__ __forceinline__ float heav_literal(float phi, float eps) {
 #pragma clang fp contract(off)
     return (float)(0.5 * (1.0 + (2.0 / kPi) * (double)atanf_literal(phi / eps)));
 }
+// The literal Heaviside with phi / eps taken as phi * r when r = 1/eps is exact:
+// for eps a power of two whose reciprocal is a normal float, both round the
+// exact value phi * 2^-k once, so they give the same float for every phi
+// (subnormal results and overflow included).  r = 0 keeps the division.
+__device__ __forceinline__ float heav_literal_r(float phi, float eps, float r) {
+#pragma clang fp contract(off)
+    const float q = r != 0.0f ? phi * r : phi / eps;
+    return (float)(0.5 * (1.0 + (2.0 / kPi) * (double)atanf_literal(q)));
+}
 // Dirac of kernel.cu:275-276: float quotient scaled by the double 1/pi.
 __device__ __forceinline__ float dirac_literal(float phi, float eps) {
 #pragma clang fp contract(off)

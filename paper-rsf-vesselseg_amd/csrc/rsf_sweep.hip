@@ -17,6 +17,7 @@
 // Both sweeps walk a tile list (persistent workgroups), so the same kernels
 // serve one volume or many uneven blocks.
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 #include <cstdio>
 
@@ -1218,7 +1219,7 @@ __device__ __forceinline__ void sweep2_voxel(const float* __restrict__ f, const 
     const float d = LIT ? dirac_literal(p0, eps) : dirac_fast(p0, eps);
     const float p1 = phi_update<LIT>(p0, d, F[i], kap, lap, dt, nu * dt, mu);
     phi_out[i] = p1;
-    tr_out[i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+    tr_out[i] = LIT ? heav_literal_r(p1, eps, inv_eps) : heav_centred(p1, inv_eps);
 }
 
 // Sweep 2, boundary shell: voxels within 2 of a face of their block, and,
@@ -1353,7 +1354,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_inner_kernel(
             const float lap = half_sum3(gRx.x - gLx.x, gRy.y - gLy.y, gRz.z - gLz.z);
             const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
             const float p1 = phi_update<LIT>(c0, d, r[19], kap, lap, dt, nu * dt, mu);
-            const float t1 = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+            const float t1 = LIT ? heav_literal_r(p1, eps, inv_eps) : heav_centred(p1, inv_eps);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, p1), rO, (int)v, 0, kNT);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, t1), rT, (int)v, 0, kNT);
         };
@@ -1599,7 +1600,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                         const float c0 = C0[i + 2];
                         const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
                         p1[i] = phi_update<LIT>(c0, d, FF[q][i], kap, lap, dt, nu * dt, mu);
-                        t1[i] = LIT ? heav_literal(p1[i], eps) : heav_centred(p1[i], inv_eps);
+                        t1[i] = LIT ? heav_literal_r(p1[i], eps, inv_eps) : heav_centred(p1[i], inv_eps);
                         if constexpr (LIT) __builtin_amdgcn_sched_barrier(0);
                     }
                 };
@@ -1867,7 +1868,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
                     const float c0 = C[k0][i + 1];
                     const float d = LIT ? dirac_literal(c0, eps) : dirac_fast(c0, eps);
                     const float p1 = phi_update<LIT>(c0, d, FF[q][i], kap, lap, dt, nu * dt, mu);
-                    const float t1 = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+                    const float t1 = LIT ? heav_literal_r(p1, eps, inv_eps) : heav_centred(p1, inv_eps);
                     po[i] = __builtin_bit_cast(uint32_t, p1);
                     pt[i] = __builtin_bit_cast(uint32_t, t1);
                     if constexpr (LIT) __builtin_amdgcn_sched_barrier(0);
@@ -1983,7 +1984,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
                 const float Fv = F[bd.off + i];
                 const float p1 = phi_update<LIT>(c0, d, Fv, kap, lap, dt, nu * dt, mu);
                 phi_out[bd.off + i] = p1;
-                tr_out[bd.off + i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+                tr_out[bd.off + i] = LIT ? heav_literal_r(p1, eps, inv_eps) : heav_centred(p1, inv_eps);
                 continue;
             }
             int qlx = lxx, qly = lxy, qlz = z + lxz_off;
@@ -2021,7 +2022,7 @@ __global__ __launch_bounds__(TX * TY) void sweep2_kernel(
             const float Fv = F[bd.off + i];
             const float p1 = phi_update<LIT>(p0, d, Fv, kap, lap, dt, nu * dt, mu);
             phi_out[bd.off + i] = p1;
-            tr_out[bd.off + i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+            tr_out[bd.off + i] = LIT ? heav_literal_r(p1, eps, inv_eps) : heav_centred(p1, inv_eps);
         }
     }
 }
@@ -2095,7 +2096,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
             const int64_t i = bd.off + (int64_t)z * plane + (int64_t)y * w + x;
             const float p1 = phi_update<LIT>(p0, d, F[i], kap, lap, dt, nu * dt, mu);
             phi_out[i] = p1;
-            tr_out[i] = LIT ? heav_literal(p1, eps) : heav_centred(p1, inv_eps);
+            tr_out[i] = LIT ? heav_literal_r(p1, eps, inv_eps) : heav_centred(p1, inv_eps);
         };
 
         load_plane(s_begin);
@@ -2179,7 +2180,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
 template <bool LIT, bool V4>
 __global__ void transform_kernel(float* __restrict__ tr, const float* __restrict__ phi, int64_t n, float eps,
                                  float inv_eps) {
-    auto T = [&](float p) { return LIT ? heav_literal(p, eps) : heav_centred(p, inv_eps); };
+    auto T = [&](float p) { return LIT ? heav_literal_r(p, eps, inv_eps) : heav_centred(p, inv_eps); };
     const int64_t step = (int64_t)gridDim.x * blockDim.x;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t done = 0;
@@ -2639,10 +2640,22 @@ int launch_blur_image(const TilePlan& tp, const void* img, int u8, float* tr, fl
     return RSF_OK;
 }
 
+// The reciprocal argument of the sweep-2 and transform kernels.  Fast mode: 1/eps.
+// Literal mode: 1/eps when eps is a power of two whose reciprocal is a normal
+// float (the kernels then multiply instead of dividing, with the same bits),
+// else 0 (they divide).
+static float eps_reciprocal(float eps, bool literal) {
+    if (!literal) return 1.0f / eps;
+    int e = 0;
+    const float m = std::frexp(eps, &e);   // eps = m * 2^e, m in [0.5, 1)
+    // eps = 2^(e-1): eps and 1/eps = 2^(1-e) both normal for e in [-125, 127]
+    return (std::isfinite(eps) && m == 0.5f && e >= -125 && e <= 127) ? 1.0f / eps : 0.0f;
+}
+
 int launch_sweep2(const TilePlan& tp, const SweepArgs& a, const rsf_params* p, hipStream_t s) {
     const Sweep2Variant& v = sweep2_variants()[tp.s2];
     const int ntiles = (int)tp.tiles2.size();
-    const float inv_eps = 1.0f / p->epsilon;
+    const float inv_eps = eps_reciprocal(p->epsilon, p->mode == RSF_MODE_LITERAL);
     void* args[] = {(void*)&a.phi,  (void*)&a.F,     (void*)&a.phi_out, (void*)&a.tr_out,
                     (void*)&tp.d_blocks.p, (void*)&tp.d_tiles2.p, (void*)&ntiles, (void*)&tp.zc2,
                     (void*)&p->dt,  (void*)&p->nu,   (void*)&p->mu,     (void*)&p->epsilon,
@@ -2673,7 +2686,7 @@ int launch_transform(float* tr, const float* phi, int64_t n, int literal, float 
     const int64_t g = std::max<int64_t>(1, std::min<int64_t>((v4 ? n / 4 + 255 : n + 255) / 256, 1 << 20));
     const void* fn = literal ? (v4 ? (const void*)&transform_kernel<true, true> : (const void*)&transform_kernel<true, false>)
                              : (v4 ? (const void*)&transform_kernel<false, true> : (const void*)&transform_kernel<false, false>);
-    const float inv_eps = 1.0f / eps;
+    const float inv_eps = eps_reciprocal(eps, literal != 0);
     void* args[] = {(void*)&tr, (void*)&phi, (void*)&n, (void*)&eps, (void*)&inv_eps};
     RSF_HIP(hipLaunchKernel(fn, dim3((unsigned)g), dim3(256), args, 0, s));
     RSF_HIP(hipGetLastError());
