@@ -1034,17 +1034,22 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
                 st(r0, z, valid ? d[0] : 30.0f);
                 return;
             }
-            float fo = 1.0f, fi = 1.0f, bH = 30.0f, bHc = 30.0f;  // pad band (kernel.cu:313, main.cpp:277)
-            if (valid) {
-                if constexpr (LIT) {
-                    bH = d[1];
-                    bHc = d[3];
-                    fo = d[0] / bH;     // kernel.cu:203
-                    fi = d[2] / bHc;
-                } else {
-                    fast_fits(bI, d[0], d[1], halfS3, fo, fi, bH, bHc);
-                }
+            // the ring's fits, or the pad band's (kernel.cu:313, main.cpp:277), by
+            // selection: a branch on the lane's validity inside the unrolled ring
+            // would make the compiler copy the ring's registers around it
+            float fo, fi, bH, bHc;
+            if constexpr (LIT) {
+                bH = d[1];
+                bHc = d[3];
+                fo = d[0] / bH;     // kernel.cu:203
+                fi = d[2] / bHc;
+            } else {
+                fast_fits(bI, d[0], d[1], halfS3, fo, fi, bH, bHc);
             }
+            fo = valid ? fo : 1.0f;
+            fi = valid ? fi : 1.0f;
+            bH = valid ? bH : 30.0f;
+            bHc = valid ? bHc : 30.0f;
             if constexpr (MODE == 0 || MODE == 3) {
                 st(r0, z, force_of(Iv, fo, fi, l1, l2, MODE == 0));
             } else {
@@ -1056,16 +1061,30 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
                 }
             }
         };
-        auto band = [&](int z) {
+        // pad-band planes [za, zb) of the column, in batches whose image loads are
+        // issued before their stores (the compiler keeps a load after a store to
+        // another buffer in order: plane by plane, every plane would wait a round trip)
+        auto band = [&](int za, int zb) {
+            constexpr int BB = 16;
             const float zero[CH] = {};
-            store_out(z, MODE == 4 ? 0.f : ldi(z), false, 0.f, zero);
+            for (int z = za; z < zb; z += BB) {
+                float iv[BB];
+#pragma unroll
+                for (int k = 0; k < BB; ++k) iv[k] = MODE != 4 && z + k < zb ? ldi(z + k) : 0.f;
+#pragma unroll
+                for (int k = 0; k < BB; ++k)
+                    if (z + k < zb) store_out(z + k, iv[k], false, 0.f, zero);
+            }
         };
+        // columns outside the valid xy region are pad band on every plane: a wave
+        // of such columns only writes the band; a wave with valid columns takes its
+        // band columns through the ring (their loads are in bounds, their sums unused)
         const bool xyvalid = x >= M && x < w - M && y >= M && y < h - M;
-        if (!xyvalid) {
-            for (int z = 0; z < l; ++z) band(z);
+        if (__ballot(xyvalid) == 0) {
+            band(0, l);
             continue;
         }
-        for (int z = 0; z < M; ++z) band(z);
+        band(0, M);
         float acc[CHL][P];
 #pragma unroll
         for (int ch = 0; ch < CHL; ++ch)
@@ -1121,7 +1140,7 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
                     d[ch] = acc[ch][js];
                     acc[ch][js] = 0.f;
                 }
-                if (!GUARD || t >= P - 1) store_out(t - (P - 1) + M, vi, true, vB, d);
+                if (!GUARD || t >= P - 1) store_out(t - (P - 1) + M, vi, xyvalid, vB, d);
             }
         };
         // queue fill: planes 0 .. D-1 (their completed planes exist only from P-1 on)
@@ -1140,7 +1159,7 @@ __global__ __launch_bounds__(TXC * TYC) __attribute__((amdgpu_waves_per_eu(4))) 
         int t0 = P;
         for (; t0 + P + D <= l; t0 += P) group(t0, std::false_type{});
         for (; t0 < l; t0 += P) group(t0, std::true_type{});
-        for (int z = l - M; z < l; ++z) band(z);
+        band(l - M, l);
     }
 }
 
