@@ -274,6 +274,19 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
                    bool aligned, int* s1, int* s2, int* zc1, int* zc2) {
     *s1 = default_sweep1(d.P, literal);
     *s2 = default_sweep2(literal);
+    // a strip xy kernel wastes the lanes past a block's valid columns: blocks
+    // narrower than three strips take the first tiled variant
+    if (*s1 >= 0 && sweep1_variants()[*s1].strip > 0) {
+        bool narrow = false;
+        for (const auto& b : blocks) narrow = narrow || b.w - 2 * d.m < 3 * sweep1_variants()[*s1].tx;
+        for (size_t k = 0; narrow && k < sweep1_variants().size(); ++k) {
+            const auto& v = sweep1_variants()[k];
+            if (v.P == d.P && v.literal == literal && v.strip == 0) {
+                *s1 = (int)k;
+                break;
+            }
+        }
+    }
     const int lmax = max_dim_l(blocks);
     int64_t vox = 0;
     for (const auto& b : blocks) vox += (int64_t)b.w * b.h * b.l;
@@ -291,8 +304,11 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
     // split sweep 1: the xy kernel has no z state, short chunks balance its work items;
     // from 2^28 voxels on, longer chunks measured faster (32x60 literal tiles at
     // 1024x1024x768: 15.0 ms with 128 planes, 15.4 with 64, 16.4 with 32; at 512^3
-    // 32 planes win)
-    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) *zc1 = std::min(lmax, vox >= (int64_t(1) << 28) ? 128 : 32);
+    // 32 planes win); a strip xy kernel takes one plane per wave of a work item
+    auto split_zc1 = [&](const Sweep1Variant& v) {
+        return v.strip > 0 ? std::min(lmax, v.strip) : std::min(lmax, vox >= (int64_t(1) << 28) ? 128 : 32);
+    };
+    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) *zc1 = split_zc1(sweep1_variants()[*s1]);
     *zc2 = *s2 >= 0 ? std::min(chunk(sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, sweep2_variants()[*s2].ty,
                                      4, 16), 64)
                     : std::min(lmax, 64);
@@ -309,6 +325,7 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
             if (a >= 0 && sweep1_variants()[a].P == d.P && sweep1_variants()[a].literal == literal) {
                 *s1 = a;
                 if (zc > 0) *zc1 = zc;
+                else if (sweep1_variants()[a].fn2) *zc1 = split_zc1(sweep1_variants()[a]);
             }
         } else {
             const int b = find_variant2(name);
@@ -1193,9 +1210,11 @@ int rsf_autotune(rsf_context* c, const rsf_block* blocks, int nblocks, int image
     const auto& V2 = sweep2_variants();
     float best1 = 1e30f, best2 = 1e30f;
     TuneChoice tc;
-    std::vector<int> zcs = {32, 64, 128, 256, lmax};
     for (size_t k = 0; k < V1.size(); ++k) {
         if (V1[k].P != d.P || V1[k].literal != literal) continue;
+        // strip xy kernels: a plane or two per wave of a work item
+        const std::vector<int> zcs = V1[k].strip > 0 ? std::vector<int>{V1[k].strip, 2 * V1[k].strip}
+                                                     : std::vector<int>{32, 64, 128, 256, lmax};
         for (int zc : zcs) {
             if (zc > lmax) continue;
             TilePlan tp;

@@ -122,6 +122,7 @@ struct Sweep1Variant {
     const void* fn2s[10];  // z kernel by 2 * output mode + (image is uint8)
     const void* fn_img[2]; // split variants: the xy kernel by (image is uint8), else null
     const void* fn_blur[2];// split fast variants: the image-only xy kernel by (image is uint8), else null
+    int strip = 0;         // strip xy kernel: waves per workgroup, each a plane of the work item's chunk
 };
 struct Sweep2Variant {
     int literal;

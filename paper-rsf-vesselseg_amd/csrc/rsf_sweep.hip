@@ -768,6 +768,180 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     }
 }
 
+// xy kernel over column strips (literal mode): each wave blurs one strip of SW =
+// 16 NCOL output columns of one plane along x and y, marching the plane's rows.
+// Per input row the wave stages the channel values (sweep1_channels) of the
+// strip's SW + P - 1 elements in LDS; lane (channel ch, column group cg) runs the
+// x pass of its NCOL columns from a register window and adds the outputs to a
+// register ring of P accumulators per column, the z kernel's ring along y:
+// accumulator j holds the output whose first row is j (mod P), row t adds tap
+// g[(t - j) mod P] to it -- starting it with the product at t = j -- and
+// completes accumulator (t + 1) mod P, which sums g[0] x_0 + ... + g[P-1] x_(P-1)
+// in order, as the tiled y pass does.  Every x-pass row is computed once (a
+// tile of TY rows recomputes P - 1 halo rows) and the waves run without
+// barriers, each in its own LDS.  Output row t - M completes at input row t.
+// The output is the interleaved layout (the four channels of a voxel in 16
+// bytes), transposed through LDS so that each lane stores one voxel.  Work
+// items: (block, strip at column x0, first plane z0), waves taking planes z0 +
+// wave, z0 + wave + NW, ... below z0 + zc.  D rows of loads stay in flight.
+template <int P, int NCOL, int NW, int D, int WPE, int IMG>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_strip_kernel(
+    const float* __restrict__ tr, const void* __restrict__ img, int img_u8_arg, float* __restrict__ xy,
+    int64_t stride, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc,
+    Taps taps) {
+    constexpr int CH = 4;
+    constexpr int M = (P - 1) / 2;
+    constexpr int NCG = 64 / CH;              // column groups per wave
+    constexpr int SW = NCG * NCOL;            // output columns per strip
+    constexpr int SE = SW + P - 1;            // input elements per strip row
+    constexpr int NE = (SE + 63) / 64;        // staged elements per lane
+    constexpr int NWIN = NCOL + P - 1;        // a lane's x-pass window
+    constexpr int VW = NCOL % 4 == 0 ? 4 : 2; // LDS read width of the window
+    constexpr int NV = (NWIN + VW - 1) / VW;
+    constexpr int SEP = 64 * NE;              // staged row length (every lane stages NE elements)
+    static_assert(NCOL % VW == 0 && SW <= 64 && (NCG - 1) * NCOL + NV * VW <= SEP, "strip shape");
+    static_assert(P % D == 0, "P must be a multiple of the load queue depth");
+    (void)stride;
+    const int img_u8 = IMG >= 0 ? IMG : img_u8_arg;
+    __shared__ __align__(16) float sRow[NW][CH][SEP];
+    __shared__ __align__(16) float sT[NW][SW * CH];
+    // the wave index in an SGPR: the planes, and with them the buffer resources, are wave-uniform
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ch = lane / NCG, cg = lane % NCG;
+    const uint8_t* img8 = (const uint8_t*)img;
+    const float* imgf = (const float*)img;
+    auto rsrc = [](const void* q) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(q), 0, (int)0xffffffff, 0x00020000);
+    };
+    for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
+        const Tile tl = tiles[item];
+        if (tl.b < 0) continue;
+        const BlockDesc bd = blocks[tl.b];
+        const int w = bd.w, h = bd.h, l = bd.l;
+        const int64_t off = bd.off, plane = (int64_t)w * h;
+        const int x0 = tl.x0;
+        const int zend = min(tl.z0 + zc, l);
+        // staged elements: lane + 64 e of the strip, columns x0 - M + ... clamped to
+        // the block (columns past w - 1, and elements past the strip's SE, feed only
+        // outputs that are not stored; every lane stages NE elements, without branches)
+        int gxe[NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) gxe[e] = min(x0 - M + lane + 64 * e, w - 1);
+        // the voxel this lane stores; lanes past the strip repeat its last voxel (same
+        // address and value), and columns past w - 1 store at w - 1: columns from
+        // w - M on are pad band, whose intermediate the z kernel reads into sums it
+        // discards, so the ring's stores need no branch
+        const int gxo = min(x0 + min(lane, SW - 1), w - 1);
+        for (int z = tl.z0 + wv; z < zend; z += NW) {
+            const int64_t zoff = off + (int64_t)z * plane;
+            const __amdgpu_buffer_rsrc_t rT = rsrc(tr + zoff);
+            const __amdgpu_buffer_rsrc_t rI = rsrc(img_u8 ? (const void*)(img8 + zoff) : (const void*)(imgf + zoff));
+            const __amdgpu_buffer_rsrc_t rX = rsrc(xy + 4 * zoff);
+            // row t of the strip: H and I of the lane's staged elements
+            // (raw bits: a conversion at the load would wait for it)
+            uint32_t qh[D][NE], qi[D][NE];
+            auto load_row = [&](int k, int t) {
+                const uint32_t ro = (uint32_t)t * (uint32_t)w;   // element offset of the row
+#pragma unroll
+                for (int e = 0; e < NE; ++e) {
+                    qh[k][e] = __builtin_amdgcn_raw_buffer_load_b32(rT, 4 * gxe[e], (int)(4u * ro), 0);
+                    if (img_u8)
+                        qi[k][e] = __builtin_amdgcn_raw_buffer_load_b8(rI, gxe[e], (int)ro, 0);
+                    else
+                        qi[k][e] = __builtin_amdgcn_raw_buffer_load_b32(rI, 4 * gxe[e], (int)(4u * ro), 0);
+                }
+            };
+            float acc[NCOL][P];
+#pragma unroll
+            for (int o = 0; o < NCOL; ++o)
+#pragma unroll
+                for (int j = 0; j < P; ++j) acc[o][j] = 0.f;
+            // groups of P rows, every step run: rows past h - 1 load row h - 1 and
+            // store at row h - 1 (output rows from h - M on are pad band, whose
+            // intermediate the z kernel reads into sums it discards); without
+            // early exits the unrolled group keeps the ring in fixed registers
+            auto group = [&](int t0, auto first_) __attribute__((always_inline)) {
+                constexpr bool FIRST = decltype(first_)::value;
+#pragma unroll
+                for (int s = 0; s < P; ++s) {
+                    const int t = t0 + s;
+                    const int k = s % D;
+                    // stage row t's channel values
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) {
+                        float cv[CH];
+                        const float iv = img_u8 ? (float)qi[k][e] : __uint_as_float(qi[k][e]);
+                        sweep1_channels<true>(iv, __uint_as_float(qh[k][e]), cv);
+#pragma unroll
+                        for (int c = 0; c < CH; ++c) sRow[wv][c][lane + 64 * e] = cv[c];
+                    }
+                    load_row(k, min(t + D, h - 1));
+                    __builtin_amdgcn_wave_barrier();
+                    // x pass of the lane's NCOL columns, each a sequential chain over
+                    // the taps (g[0] x_0 first).  The window streams through in reads of
+                    // VW elements, the next read in flight while the current one is
+                    // used; scheduling barriers keep the compiler from hoisting all the
+                    // reads (their registers would spill the ring)
+                    float xo[NCOL];
+                    const float* src = &sRow[wv][ch][cg * NCOL];
+                    auto rd = [&](int q, float (&v)[VW]) {
+                        if constexpr (VW == 4) {
+                            const float4 f = *reinterpret_cast<const float4*>(src + 4 * q);
+                            v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+                        } else {
+                            const float2 f = *reinterpret_cast<const float2*>(src + 2 * q);
+                            v[0] = f.x; v[1] = f.y;
+                        }
+                    };
+                    float vc[VW], vn[VW];
+                    rd(0, vc);
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) {
+                        if (q + 1 < NV) rd(q + 1, vn);
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int jj = 0; jj < VW; ++jj) {
+                            const int el = VW * q + jj;
+#pragma unroll
+                            for (int o = 0; o < NCOL; ++o) {
+                                const int kk = el - o;
+                                if (kk == 0) xo[o] = vc[jj] * taps.g[0];
+                                else if (kk > 0 && kk < P) xo[o] = fmaf(vc[jj], taps.g[kk], xo[o]);
+                            }
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int jj = 0; jj < VW; ++jj) vc[jj] = vn[jj];
+                    }
+                    // y ring: accumulator s starts with its first product, the others add
+                    // tap (s - j) mod P; accumulator (s + 1) mod P completes
+#pragma unroll
+                    for (int j = 0; j < P; ++j) {
+#pragma unroll
+                        for (int o = 0; o < NCOL; ++o)
+                            acc[o][j] = j == s ? xo[o] * taps.g[0] : fmaf(xo[o], taps.g[(s - j + P) % P], acc[o][j]);
+                    }
+                    if (!FIRST || s >= P - 1) {
+                        const int js = (s + 1) % P;
+#pragma unroll
+                        for (int o = 0; o < NCOL; ++o) sT[wv][(cg * NCOL + o) * CH + ch] = acc[o][js];
+                        __builtin_amdgcn_wave_barrier();
+                        const float4 vv = *reinterpret_cast<const float4*>(&sT[wv][min(lane, SW - 1) * CH]);
+                        const u32x4 b = {__float_as_uint(vv.x), __float_as_uint(vv.y), __float_as_uint(vv.z),
+                                         __float_as_uint(vv.w)};
+                        __builtin_amdgcn_raw_buffer_store_b128(b, rX, 16 * gxo,
+                                                               (int)(16u * (uint32_t)min(t - M, h - 1) * (uint32_t)w), kNT);
+                    }
+                }
+            };
+#pragma unroll
+            for (int k = 0; k < D; ++k) load_row(k, min(k, h - 1));
+            group(0, std::true_type{});
+            for (int t0 = P; t0 < h; t0 += P) group(t0, std::false_type{});
+        }
+    }
+}
+
 // FAST xy kernel on the matrix cores.  The x pass of 16 rows x 16 outputs is
 // the product of the 16 x 36 input window with the 36 x 16 band matrix
 // T[m][j] = g[m - j] (0 <= m - j < P), nine v_mfma_f32_16x16x4_f32; the y pass
@@ -2292,6 +2466,18 @@ Sweep1Variant s1s(const char* name) {
     }
     return v;
 }
+// split sweep 1, literal mode: the strip xy kernel (NW waves per workgroup, one
+// plane per wave at a time) and the interleaved z kernel
+template <int P, int NCOL, int NW, int D, int WPE, int TXC, int DZ>
+Sweep1Variant s1strip(const char* name) {
+    Sweep1Variant v = s1split<P, true, 16 * NCOL, 1 << 20, 64 * NW, TXC, 1, DZ, true>(
+        (const void*)&sweep1_xy_strip_kernel<P, NCOL, NW, D, WPE, 1>, name);
+    v.fn_img[0] = (const void*)&sweep1_xy_strip_kernel<P, NCOL, NW, D, WPE, 0>;
+    v.fn_img[1] = (const void*)&sweep1_xy_strip_kernel<P, NCOL, NW, D, WPE, 1>;
+    v.max_plane = (int64_t)(0xffffffffu / (16u * (P + DZ)));
+    v.strip = NW;
+    return v;
+}
 template <int P, int TX, int NT, int WPE, int TXC, int TYC, int D>
 Sweep1Variant s1m(const char* name) {
     Sweep1Variant v = s1split<P, false, TX, 28, NT, TXC, TYC, D>(
@@ -2311,10 +2497,16 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         // x and y passes on the matrix cores: a tuner candidate, 1.56 ms per sweep 1 at
         // 512^3 against 1.36 for the default (MFMA at about a third of its rate)
         s1m<21, 64, 256, 2, 256, 1, 3>("p21_fast_mfma_64x28"),
-        // literal default: 32x60 tiles, 15 outputs per y item, each x-pass item forming
-        // all four channels from one H/I window, the intermediate interleaved per voxel
-        // (sweep 1 at 512^3: 2.05 ms; 2.20 with channel planes, 2.24 with channel pairs
-        // per x-pass item)
+        // literal default: the strip xy kernel, 64-column strips, a plane per wave (xy
+        // kernel at 512^3: 0.82 ms against 1.02 for the 32x60 tiles below), with the
+        // interleaved z kernel
+        s1strip<21, 4, 4, 1, 4, 256, 3>("p21_lit_strip_64w4"),
+        // the literal default for blocks narrower than three strips (64 uneven blocks of
+        // sides 96 to 256: 3% faster than the strips): 32x60 tiles, 15 outputs per y
+        // item, each x-pass item forming all four channels from one H/I window, the
+        // intermediate interleaved per voxel (sweep 1 at 512^3: 2.05 ms before the z
+        // kernel's band rework; 2.20 with channel planes, 2.24 with channel pairs per
+        // x-pass item)
         s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, 2, 4, 1, 1, true>("p21_lit_il_32x60r15"),
         // the same on channel planes: the default where a plane is too large for the
         // interleaved z kernel's 32-bit offsets
@@ -2329,6 +2521,12 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true, 8>("p21_lit_hi_32x44x8"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, 2, 8>("p21_lit_q4_32x44x8"),
+        // more strip kernels: 3 rows of loads in flight, 3 waves per SIMD, 32-column
+        // strips (xy kernel at 512^3: 0.82, 0.83, 0.98 and 1.01 ms)
+        s1strip<21, 4, 4, 3, 4, 256, 3>("p21_lit_strip_64w4d3"),
+        s1strip<21, 4, 4, 3, 3, 256, 3>("p21_lit_strip_64w3"),
+        s1strip<21, 2, 4, 3, 7, 256, 3>("p21_lit_strip_32w7"),
+        s1strip<21, 2, 4, 1, 8, 256, 3>("p21_lit_strip_32w8"),
         // more tiles on the interleaved intermediate
         s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, 1, 4, 1, 1, true>("p21_lit_ilhi_32x60r15"),
         s1s<21, true, 32, 28, 256, 7, 4, 256, 1, 3>("p21_lit_split_32x28w4"),

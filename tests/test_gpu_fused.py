@@ -232,7 +232,8 @@ def _registered(mode, P=21):
     return s1, s2
 
 
-@pytest.mark.parametrize("shape", [(40, 44, 48), (52, 97, 130)], ids=["40x44x48", "52x97x130"])
+@pytest.mark.parametrize("shape", [(40, 44, 48), (52, 97, 130), (37, 131, 236)],
+                         ids=["40x44x48", "52x97x130", "37x131x236"])
 @pytest.mark.parametrize("lambdas", [(1.0, 1.0), (1.3, 0.7)])
 @pytest.mark.parametrize("dtype", [np.uint8, np.float32])
 @pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
@@ -245,8 +246,11 @@ def test_kernel_variants_give_identical_results(ctx, dtype, mode, lambdas, shape
     mode's matrix-core xy kernels sum the blur in another order: they are held
     to the fast-mode gate against the oracle (3 steps within 1e-4 max|phi|).  The second
     shape has full and partial tiles and rows that are not 16-byte aligned (w =
-    130), the first aligned rows (the vector sweep-2 interior).  In literal mode
-    the default run is also the oracle's phi bit for bit."""
+    130), the first aligned rows (the vector sweep-2 interior); the third four
+    64-column strips, the last partial, and a row count (131) that is not a
+    multiple of the strip kernel's 21-row groups, with the strip kernel as the
+    literal default (wide enough for it).  In literal mode the default run is
+    also the oracle's phi bit for bit."""
     import torch
 
     img, phi0, _ = scene(shape, tubes=6, seed=23)

@@ -228,7 +228,10 @@ def test_variant_query_names_the_tuner_candidates():
     35, and sweep-2 kernels of both modes; out-of-range queries are errors."""
     v1, v2 = rsf_amd.variants(1), rsf_amd.variants(2)
     names1 = {n for n, _, _ in v1}
-    assert {"p21_lit_hi_32x44x8", "p21_lit_hi_32x60r15", "p21_fast_split_64x44w4"} <= names1
+    assert {"p21_lit_hi_32x44x8", "p21_lit_hi_32x60r15", "p21_fast_split_64x44w4", "p21_lit_strip_64w4",
+            "p21_lit_strip_64w3"} <= names1
+    # the literal default (the first literal P = 21 entry) is the strip kernel
+    assert next(n for n, t, l in v1 if t == 21 and l) == "p21_lit_strip_64w4"
     for P in range(7, 37, 2):
         for lit in (False, True):
             assert any(t == P and l == lit for _, t, l in v1), (P, lit)
