@@ -768,8 +768,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     }
 }
 
-// xy kernel over column strips (literal mode): each wave blurs one strip of SW =
-// 16 NCOL output columns of one plane along x and y, marching the plane's rows.
+// xy kernel over column strips: each wave blurs one strip of SW = (64 / CH) NCOL
+// output columns of one plane along x and y, marching the plane's rows.
 // Per input row the wave stages the channel values (sweep1_channels) of the
 // strip's SW + P - 1 elements in LDS; lane (channel ch, column group cg) runs the
 // x pass of its NCOL columns from a register window and adds the outputs to a
@@ -780,16 +780,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 // in order, as the tiled y pass does.  Every x-pass row is computed once (a
 // tile of TY rows recomputes P - 1 halo rows) and the waves run without
 // barriers, each in its own LDS.  Output row t - M completes at input row t.
-// The output is the interleaved layout (the four channels of a voxel in 16
-// bytes), transposed through LDS so that each lane stores one voxel.  Work
+// Literal mode writes the interleaved layout (the four channels of a voxel in 16
+// bytes), transposed through LDS so that each lane stores one voxel; fast mode
+// the two channel planes, each lane its NCOL columns.  Work
 // items: (block, strip at column x0, first plane z0), waves taking planes z0 +
 // wave, z0 + wave + NW, ... below z0 + zc.  D rows of loads stay in flight.
-template <int P, int NCOL, int NW, int D, int WPE, int IMG>
+template <int P, bool LIT, int NCOL, int NW, int D, int WPE, int IMG>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void sweep1_xy_strip_kernel(
     const float* __restrict__ tr, const void* __restrict__ img, int img_u8_arg, float* __restrict__ xy,
     int64_t stride, const BlockDesc* __restrict__ blocks, const Tile* __restrict__ tiles, int ntiles, int zc,
     Taps taps) {
-    constexpr int CH = 4;
+    constexpr int CH = LIT ? 4 : 2;
     constexpr int M = (P - 1) / 2;
     constexpr int NCG = 64 / CH;              // column groups per wave
     constexpr int SW = NCG * NCOL;            // output columns per strip
@@ -799,12 +800,12 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
     constexpr int VW = NCOL % 4 == 0 ? 4 : 2; // LDS read width of the window
     constexpr int NV = (NWIN + VW - 1) / VW;
     constexpr int SEP = 64 * NE;              // staged row length (every lane stages NE elements)
-    static_assert(NCOL % VW == 0 && SW <= 64 && (NCG - 1) * NCOL + NV * VW <= SEP, "strip shape");
-    static_assert(P % D == 0, "P must be a multiple of the load queue depth");
-    (void)stride;
+    static_assert(NCOL % VW == 0 && (SW <= 64 || !LIT) && (NCG - 1) * NCOL + NV * VW <= SEP, "strip shape");
+    static_assert(P % D == 0 && NCOL <= M, "P must be a multiple of the load queue depth");
+    typedef float fvec __attribute__((ext_vector_type(NCOL), aligned(4)));   // a lane's columns (fast mode)
     const int img_u8 = IMG >= 0 ? IMG : img_u8_arg;
     __shared__ __align__(16) float sRow[NW][CH][SEP];
-    __shared__ __align__(16) float sT[NW][SW * CH];
+    __shared__ __align__(16) float sT[LIT ? NW : 1][LIT ? SW * CH : 1];
     // the wave index in an SGPR: the planes, and with them the buffer resources, are wave-uniform
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int ch = lane / NCG, cg = lane % NCG;
@@ -832,6 +833,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
         // w - M on are pad band, whose intermediate the z kernel reads into sums it
         // discards, so the ring's stores need no branch
         const int gxo = min(x0 + min(lane, SW - 1), w - 1);
+        // fast mode: the lane's first column, groups past w - NCOL at w - NCOL (all
+        // their columns are pad band, M >= NCOL)
+        const int gxc = min(x0 + cg * NCOL, w - NCOL);
         for (int z = tl.z0 + wv; z < zend; z += NW) {
             const int64_t zoff = off + (int64_t)z * plane;
             const __amdgpu_buffer_rsrc_t rT = rsrc(tr + zoff);
@@ -871,7 +875,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
                     for (int e = 0; e < NE; ++e) {
                         float cv[CH];
                         const float iv = img_u8 ? (float)qi[k][e] : __uint_as_float(qi[k][e]);
-                        sweep1_channels<true>(iv, __uint_as_float(qh[k][e]), cv);
+                        sweep1_channels<LIT>(iv, __uint_as_float(qh[k][e]), cv);
 #pragma unroll
                         for (int c = 0; c < CH; ++c) sRow[wv][c][lane + 64 * e] = cv[c];
                     }
@@ -923,18 +927,25 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
                     }
                     if (!FIRST || s >= P - 1) {
                         const int js = (s + 1) % P;
+                        const uint32_t yo = (uint32_t)min(t - M, h - 1);
+                        if constexpr (LIT) {
 #pragma unroll
-                        for (int o = 0; o < NCOL; ++o) sT[wv][(cg * NCOL + o) * CH + ch] = acc[o][js];
-                        __builtin_amdgcn_wave_barrier();
-                        const float4 vv = *reinterpret_cast<const float4*>(&sT[wv][min(lane, SW - 1) * CH]);
-                        const u32x4 b = {__float_as_uint(vv.x), __float_as_uint(vv.y), __float_as_uint(vv.z),
-                                         __float_as_uint(vv.w)};
-                        __builtin_amdgcn_raw_buffer_store_b128(b, rX, 16 * gxo,
-                                                               (int)(16u * (uint32_t)min(t - M, h - 1) * (uint32_t)w), kNT);
+                            for (int o = 0; o < NCOL; ++o) sT[wv][(cg * NCOL + o) * CH + ch] = acc[o][js];
+                            __builtin_amdgcn_wave_barrier();
+                            const float4 vv = *reinterpret_cast<const float4*>(&sT[wv][min(lane, SW - 1) * CH]);
+                            const u32x4 b = {__float_as_uint(vv.x), __float_as_uint(vv.y), __float_as_uint(vv.z),
+                                             __float_as_uint(vv.w)};
+                            __builtin_amdgcn_raw_buffer_store_b128(b, rX, 16 * gxo, (int)(16u * yo * (uint32_t)w),
+                                                                   kNT);
+                        } else {
+                            fvec v;
+#pragma unroll
+                            for (int o = 0; o < NCOL; ++o) v[o] = acc[o][js];
+                            *reinterpret_cast<fvec*>(xy + ch * stride + zoff + (int64_t)yo * w + gxc) = v;
+                        }
                     }
                 }
             };
-#pragma unroll
             for (int k = 0; k < D; ++k) load_row(k, min(k, h - 1));
             group(0, std::true_type{});
             for (int t0 = P; t0 < h; t0 += P) group(t0, std::false_type{});
@@ -2466,15 +2477,15 @@ Sweep1Variant s1s(const char* name) {
     }
     return v;
 }
-// split sweep 1, literal mode: the strip xy kernel (NW waves per workgroup, one
-// plane per wave at a time) and the interleaved z kernel
-template <int P, int NCOL, int NW, int D, int WPE, int TXC, int DZ>
+// split sweep 1: the strip xy kernel (NW waves per workgroup, one plane per wave
+// at a time) and the z kernel (literal mode: on the interleaved intermediate)
+template <int P, bool LIT, int NCOL, int NW, int D, int WPE, int TXC, int DZ>
 Sweep1Variant s1strip(const char* name) {
-    Sweep1Variant v = s1split<P, true, 16 * NCOL, 1 << 20, 64 * NW, TXC, 1, DZ, true>(
-        (const void*)&sweep1_xy_strip_kernel<P, NCOL, NW, D, WPE, 1>, name);
-    v.fn_img[0] = (const void*)&sweep1_xy_strip_kernel<P, NCOL, NW, D, WPE, 0>;
-    v.fn_img[1] = (const void*)&sweep1_xy_strip_kernel<P, NCOL, NW, D, WPE, 1>;
-    v.max_plane = (int64_t)(0xffffffffu / (16u * (P + DZ)));
+    Sweep1Variant v = s1split<P, LIT, (LIT ? 16 : 32) * NCOL, 1 << 20, 64 * NW, TXC, 1, DZ, LIT>(
+        (const void*)&sweep1_xy_strip_kernel<P, LIT, NCOL, NW, D, WPE, 1>, name);
+    v.fn_img[0] = (const void*)&sweep1_xy_strip_kernel<P, LIT, NCOL, NW, D, WPE, 0>;
+    v.fn_img[1] = (const void*)&sweep1_xy_strip_kernel<P, LIT, NCOL, NW, D, WPE, 1>;
+    if constexpr (LIT) v.max_plane = (int64_t)(0xffffffffu / (16u * (P + DZ)));
     v.strip = NW;
     return v;
 }
@@ -2490,6 +2501,10 @@ Sweep1Variant s1m(const char* name) {
 
 const std::vector<Sweep1Variant>& sweep1_variants() {
     static const std::vector<Sweep1Variant> v = {
+        // fast default: the strip xy kernel, 128-column strips, a plane per wave (xy
+        // kernel at 512^3: 0.49 ms against 0.60 for the 64x44 tiles below)
+        s1strip<21, false, 4, 4, 1, 4, 256, 3>("p21_fast_strip_128w4"),
+        // the fast default for blocks narrower than three strips
         s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, false, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
         s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
@@ -2497,10 +2512,13 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         // x and y passes on the matrix cores: a tuner candidate, 1.56 ms per sweep 1 at
         // 512^3 against 1.36 for the default (MFMA at about a third of its rate)
         s1m<21, 64, 256, 2, 256, 1, 3>("p21_fast_mfma_64x28"),
+        // the strip xy kernel in fast mode, 64-column strips
+        s1strip<21, false, 2, 4, 3, 5, 256, 3>("p21_fast_strip_64w5"),
+        s1strip<21, false, 2, 4, 3, 4, 256, 3>("p21_fast_strip_64w4"),
         // literal default: the strip xy kernel, 64-column strips, a plane per wave (xy
         // kernel at 512^3: 0.82 ms against 1.02 for the 32x60 tiles below), with the
         // interleaved z kernel
-        s1strip<21, 4, 4, 1, 4, 256, 3>("p21_lit_strip_64w4"),
+        s1strip<21, true, 4, 4, 1, 4, 256, 3>("p21_lit_strip_64w4"),
         // the literal default for blocks narrower than three strips (64 uneven blocks of
         // sides 96 to 256: 3% faster than the strips): 32x60 tiles, 15 outputs per y
         // item, each x-pass item forming all four channels from one H/I window, the
@@ -2523,10 +2541,10 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, 2, 8>("p21_lit_q4_32x44x8"),
         // more strip kernels: 3 rows of loads in flight, 3 waves per SIMD, 32-column
         // strips (xy kernel at 512^3: 0.82, 0.83, 0.98 and 1.01 ms)
-        s1strip<21, 4, 4, 3, 4, 256, 3>("p21_lit_strip_64w4d3"),
-        s1strip<21, 4, 4, 3, 3, 256, 3>("p21_lit_strip_64w3"),
-        s1strip<21, 2, 4, 3, 7, 256, 3>("p21_lit_strip_32w7"),
-        s1strip<21, 2, 4, 1, 8, 256, 3>("p21_lit_strip_32w8"),
+        s1strip<21, true, 4, 4, 3, 4, 256, 3>("p21_lit_strip_64w4d3"),
+        s1strip<21, true, 4, 4, 3, 3, 256, 3>("p21_lit_strip_64w3"),
+        s1strip<21, true, 2, 4, 3, 7, 256, 3>("p21_lit_strip_32w7"),
+        s1strip<21, true, 2, 4, 1, 8, 256, 3>("p21_lit_strip_32w8"),
         // more tiles on the interleaved intermediate
         s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, 1, 4, 1, 1, true>("p21_lit_ilhi_32x60r15"),
         s1s<21, true, 32, 28, 256, 7, 4, 256, 1, 3>("p21_lit_split_32x28w4"),

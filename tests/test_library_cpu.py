@@ -230,8 +230,9 @@ def test_variant_query_names_the_tuner_candidates():
     names1 = {n for n, _, _ in v1}
     assert {"p21_lit_hi_32x44x8", "p21_lit_hi_32x60r15", "p21_fast_split_64x44w4", "p21_lit_strip_64w4",
             "p21_lit_strip_64w3"} <= names1
-    # the literal default (the first literal P = 21 entry) is the strip kernel
+    # the defaults (the first P = 21 entry of each mode) are the strip kernels
     assert next(n for n, t, l in v1 if t == 21 and l) == "p21_lit_strip_64w4"
+    assert next(n for n, t, l in v1 if t == 21 and not l) == "p21_fast_strip_128w4"
     for P in range(7, 37, 2):
         for lit in (False, True):
             assert any(t == P and l == lit for _, t, l in v1), (P, lit)
