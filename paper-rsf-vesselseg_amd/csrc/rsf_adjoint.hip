@@ -120,7 +120,9 @@ __global__ void k_conv_same_zero(float* out, const float* in, int w, int h, int 
 }
 
 // The same correlation for two channels at once, with the axis and the tap count
-// fixed at compile time: a 64 x 4 thread block covers 64 x-voxels of 4 rows
+// fixed at compile time (the x pass: consecutive lanes read consecutive
+// columns, which a thread's own run of columns would not): a 64 x 4 thread
+// block covers 64 x-voxels of 4 rows
 // (y + z*h), rows grid-strided, no 64-bit division; along y and z interior rows
 // take the unrolled taps without bounds checks, along x every lane runs the
 // same masked taps.  The taps are summed in the order of k_conv_same_zero.
@@ -172,23 +174,98 @@ __global__ __launch_bounds__(256) void k_conv_same_zero2(float* __restrict__ out
     }
 }
 
+// The y and z passes of k_conv_same_zero2 with R consecutive outputs along the
+// axis per thread: the R + P - 1 inputs of a thread's outputs are loaded once
+// (R + P - 1 loads for R outputs instead of R P; y 22.6 -> 10.9 us, z 20.0 -> 10.8
+// us for two 128^3 channels).  Out-of-range taps are skipped by selection, so
+// every output is the same chain as k_conv_same_zero2's.  A 64 x 4 thread block:
+// 64 x-voxels of 4 work items (the other coordinate and the group of R along the
+// axis), work items grid-strided.  (Along x, R consecutive columns per thread made
+// the pass slower, 20 -> 43 us: a wave's loads then spread over 2 KB each.)
+template <int AXIS, int P, int R>
+__global__ __launch_bounds__(256) void k_conv_same_zero2r(float* __restrict__ out0, float* __restrict__ out1,
+                                                          const float* __restrict__ in0,
+                                                          const float* __restrict__ in1, int w, int h, int l,
+                                                          Taps t) {
+    static_assert(AXIS == 1 || AXIS == 2, "y or z");
+    constexpr int m = (P - 1) / 2, NV = R + P - 1;
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    if (x >= w) return;
+    const int dim = AXIS == 1 ? h : l, other = AXIS == 1 ? l : h;
+    const int groups = (dim + R - 1) / R;
+    const int64_t plane = (int64_t)w * h;
+    const int64_t stride = AXIS == 1 ? (int64_t)w : plane;
+    const int items = groups * other;
+    for (int it = blockIdx.y * 4 + threadIdx.y; it < items; it += gridDim.y * 4) {
+        const int o = it / groups, c0 = (it - o * groups) * R;
+        const int64_t base = (AXIS == 1 ? (int64_t)o * plane : (int64_t)o * w) + x;   // c = 0 of the line
+        float v0[NV], v1[NV];
+        if (c0 >= m && c0 + R - 1 + m < dim) {
+            const float* p0 = in0 + base + (int64_t)(c0 - m) * stride;
+            const float* p1 = in1 + base + (int64_t)(c0 - m) * stride;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                v0[k] = p0[k * stride];
+                v1[k] = p1[k * stride];
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    a0 = fmaf(t.g[k], v0[r + k], a0);
+                    a1 = fmaf(t.g[k], v1[r + k], a1);
+                }
+                out0[base + (int64_t)(c0 + r) * stride] = a0;
+                out1[base + (int64_t)(c0 + r) * stride] = a1;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const int cc = min(max(c0 - m + k, 0), dim - 1);
+                v0[k] = in0[base + (int64_t)cc * stride];
+                v1[k] = in1[base + (int64_t)cc * stride];
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    const int cc = c0 + r - m + k;
+                    const bool in = cc >= 0 && cc < dim;
+                    a0 = in ? fmaf(t.g[k], v0[r + k], a0) : a0;
+                    a1 = in ? fmaf(t.g[k], v1[r + k], a1) : a1;
+                }
+                if (c0 + r < dim) {
+                    out0[base + (int64_t)(c0 + r) * stride] = a0;
+                    out1[base + (int64_t)(c0 + r) * stride] = a1;
+                }
+            }
+        }
+    }
+}
+
 template <int P>
 int conv2_axes(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w, int h, int l,
                const Taps& taps, hipStream_t s) {
+    constexpr int R = 8;
     const dim3 blk(64, 4);
     const dim3 grid((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)h * l + 3) / 4, 65535));
     hipLaunchKernelGGL((k_conv_same_zero2<0, P>), grid, blk, 0, s, Bh, Ba, QH, QA, w, h, l, taps);
-    hipLaunchKernelGGL((k_conv_same_zero2<1, P>), grid, blk, 0, s, t1, t2, (const float*)Bh, (const float*)Ba, w, h,
+    const dim3 gy((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)((h + R - 1) / R) * l + 3) / 4, 65535));
+    hipLaunchKernelGGL((k_conv_same_zero2r<1, P, R>), gy, blk, 0, s, t1, t2, (const float*)Bh, (const float*)Ba, w, h,
                        l, taps);
-    hipLaunchKernelGGL((k_conv_same_zero2<2, P>), grid, blk, 0, s, Bh, Ba, (const float*)t1, (const float*)t2, w, h,
+    const dim3 gz((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)((l + R - 1) / R) * h + 3) / 4, 65535));
+    hipLaunchKernelGGL((k_conv_same_zero2r<2, P, R>), gz, blk, 0, s, Bh, Ba, (const float*)t1, (const float*)t2, w, h,
                        l, taps);
     RSF_HIP(hipGetLastError());
     return RSF_OK;
 }
 
 // G_a at every voxel: normalisation Jacobian applied to D^T(kbar) plus D^T(Lbar).
-__global__ void k_stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* gout, int w, int h,
-                            int l, float dt, float nu, float mu, float eps) {
+// Also the forward normal and gradient at the voxel, for k_finish.
+__global__ void k_stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, const float* phi, const float* gout,
+                            int w, int h, int l, float dt, float nu, float mu, float eps) {
     const int64_t plane = (int64_t)w * h, n = plane * l;
     auto kbar = [&](int64_t q) { return (nu * dt * dirac(phi[q], eps) - mu * dt) * gout[q]; };
     auto lbar = [&](int64_t q) { return mu * dt * gout[q]; };
@@ -204,12 +281,19 @@ __global__ void k_stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, c
         Gx[j] = nbx * r - g.x * c + lbx;
         Gy[j] = nby * r - g.y * c + lby;
         Gz[j] = nbz * r - g.z * c + lbz;
+        fwd.n[0][j] = g.x / (M + kEta);
+        fwd.n[1][j] = g.y / (M + kEta);
+        fwd.n[2][j] = g.z / (M + kEta);
+        fwd.g[0][j] = g.x;
+        fwd.g[1][j] = g.y;
+        fwd.g[2][j] = g.z;
     }
 }
 
 // dphi and per-workgroup partial sums of the parameter gradients.
 __global__ __launch_bounds__(kT) void k_finish(float* gin, float* partials, const float* phi, const float* gout,
-                                               const float* Gx, const float* Gy, const float* Gz, const float* Bh,
+                                               const float* Gx, const float* Gy, const float* Gz, FwdStencil fwd,
+                                               const float* Bh,
                                                const float* Ba, const void* img, int u8, const float* fo,
                                                const float* fi, int w, int h, int l, float dt, float nu, float mu,
                                                float eps, float l1, float l2) {
@@ -222,21 +306,17 @@ __global__ __launch_bounds__(kT) void k_finish(float* gin, float* partials, cons
         int x, y, z;
         coords(i, w, h, n, x, y, z);
         const float st = dT_x(gx, i, x, w, n) + dT_yz(gy, i, y, h, w) + dT_yz(gz, i, z, l, plane);
-        // forward curvature and Laplacian at i (same stencil as sweep 2)
-        int qlx, qly, qlz;
-        left_x(x, y, z, w, h, qlx, qly, qlz);
-        const int rxx = (x == w - 1) ? x : x + 1;
-        const int ryy = (y == 0) ? 1 : (y == h - 1 ? y : y + 1);
-        const int lyy = (y == 0) ? 0 : (y == h - 1 ? h - 2 : y - 1);
-        const int rzz = (z == 0) ? 1 : (z == l - 1 ? z : z + 1);
-        const int lzz = (z == 0) ? 0 : (z == l - 1 ? l - 2 : z - 1);
-        const Grad gRx = grad_at(phi, rxx, y, z, w, h, l), gLx = grad_at(phi, qlx, qly, qlz, w, h, l);
-        const Grad gRy = grad_at(phi, x, ryy, z, w, h, l), gLy = grad_at(phi, x, lyy, z, w, h, l);
-        const Grad gRz = grad_at(phi, x, y, rzz, w, h, l), gLz = grad_at(phi, x, y, lzz, w, h, l);
-        auto nc = [](const Grad& g, float c) { return c / (sqrtf(g.x * g.x + g.y * g.y + g.z * g.z) + kEta); };
-        const float kap = ((nc(gRx, gRx.x) - nc(gLx, gLx.x)) + (nc(gRy, gRy.y) - nc(gLy, gLy.y)) +
-                           (nc(gRz, gRz.z) - nc(gLz, gLz.z))) * 0.5f;
-        const float lap = ((gRx.x - gLx.x) + (gRy.y - gLy.y) + (gRz.z - gLz.z)) * 0.5f;
+        // forward curvature and Laplacian at i (same stencil as sweep 2), from the
+        // neighbours' normals and gradients (k_stencil_g): the right x neighbour (i
+        // itself at x = w-1), the left one the previous linear element (wrap, element
+        // 0 at i = 0), y and z one-sided at the ends
+        const int64_t iRx = x == w - 1 ? i : i + 1, iLx = i == 0 ? 0 : i - 1;
+        const int64_t iRy = y == h - 1 ? i : i + w, iLy = y == 0 ? i : i - w;
+        const int64_t iRz = z == l - 1 ? i : i + plane, iLz = z == 0 ? i : i - plane;
+        const float kap = ((fwd.n[0][iRx] - fwd.n[0][iLx]) + (fwd.n[1][iRy] - fwd.n[1][iLy]) +
+                           (fwd.n[2][iRz] - fwd.n[2][iLz])) * 0.5f;
+        const float lap = ((fwd.g[0][iRx] - fwd.g[0][iLx]) + (fwd.g[1][iRy] - fwd.g[1][iLy]) +
+                           (fwd.g[2][iRz] - fwd.g[2][iLz])) * 0.5f;
         const float I = image_at(img, u8, i);
         const float d1 = I - fo[i], d2 = I - fi[i];
         const float Eo = d1 * d1, Ei = d2 * d2;
@@ -335,26 +415,28 @@ int blur_transpose2(float* Bh, float* Ba, const float* QH, const float* QA, floa
     }
 }
 
-int stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* gout, int w, int h, int l,
-              const rsf_params* p, hipStream_t s) {
+int stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, const float* phi, const float* gout, int w, int h,
+              int l, const rsf_params* p, hipStream_t s) {
     const int64_t n = (int64_t)w * h * l;
-    hipLaunchKernelGGL(k_stencil_g, grid_for(n), dim3(kT), 0, s, Gx, Gy, Gz, phi, gout, w, h, l, p->dt, p->nu, p->mu,
-                       p->epsilon);
+    hipLaunchKernelGGL(k_stencil_g, grid_for(n), dim3(kT), 0, s, Gx, Gy, Gz, fwd, phi, gout, w, h, l, p->dt, p->nu,
+                       p->mu, p->epsilon);
     RSF_HIP(hipGetLastError());
     return RSF_OK;
 }
 
 // the finishing kernel runs at most 1024 workgroups per block, each thread
-// accumulating several voxels before the reduction
+// accumulating several voxels before the reduction (4096 workgroups at 128^3
+// saved 1 us in it and cost 37 us in the parameter reduction)
 dim3 grid_finish(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kT - 1) / kT, 1024))); }
 
 int finish_rows(int64_t n) { return (int)grid_finish(n).x; }
 
 int finish(float* gin, float* partials, const float* phi, const float* gout, const float* Gx, const float* Gy,
-           const float* Gz, const float* Bh, const float* Ba, const void* img, int u8, const float* fo,
-           const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s) {
+           const float* Gz, FwdStencil fwd, const float* Bh, const float* Ba, const void* img, int u8,
+           const float* fo, const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s) {
     const int64_t n = (int64_t)w * h * l;
-    hipLaunchKernelGGL(k_finish, grid_finish(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, Bh, Ba, img, u8,
+    hipLaunchKernelGGL(k_finish, grid_finish(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, fwd, Bh, Ba,
+                       img, u8,
                        fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2);
     RSF_HIP(hipGetLastError());
     return RSF_OK;

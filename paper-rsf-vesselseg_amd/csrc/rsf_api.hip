@@ -1305,9 +1305,12 @@ int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks
         RSF_TRY(adj::pointwise_q(QH + o, QA + o, phi + o, grad_out + o, im, u8, fo + o, fi + o, PH + o, PHc + o, b.w,
                                  b.h, b.l, pr.d.m, p, s));
         RSF_TRY(adj::blur_transpose2(Bh + o, Ba + o, QH + o, QA + o, t1 + o, t2 + o, b.w, b.h, b.l, pr.d, s));
-        RSF_TRY(adj::stencil_g(Gx + o, Gy + o, Gz + o, phi + o, grad_out + o, b.w, b.h, b.l, p, s));
-        RSF_TRY(adj::finish(grad_in + o, parts + 4 * row, phi + o, grad_out + o, Gx + o, Gy + o, Gz + o, Bh + o, Ba + o,
-                            im, u8, fo + o, fi + o, b.w, b.h, b.l, p, s));
+        // the forward normals and gradients into the buffers free by now (PH, PHc:
+        // read by the point-wise step; QH, QA, t1, t2: by the transposed blur)
+        const adj::FwdStencil fwd{{PH + o, PHc + o, QH + o}, {QA + o, t1 + o, t2 + o}};
+        RSF_TRY(adj::stencil_g(Gx + o, Gy + o, Gz + o, fwd, phi + o, grad_out + o, b.w, b.h, b.l, p, s));
+        RSF_TRY(adj::finish(grad_in + o, parts + 4 * row, phi + o, grad_out + o, Gx + o, Gy + o, Gz + o, fwd, Bh + o,
+                            Ba + o, im, u8, fo + o, fi + o, b.w, b.h, b.l, p, s));
         row += adj::finish_rows((int64_t)b.w * b.h * b.l);
     }
     RSF_TRY(adj::reduce_params(grad_params, parts, row, s));

@@ -191,12 +191,19 @@ int blur_transpose(float* out, const float* in, float* t1, float* t2, int w, int
 // Both co-states at once (Bh <- blur^T QH, Ba <- blur^T QA); t1, t2 are scratch.
 int blur_transpose2(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w,
                     int h, int l, const Derived& d, hipStream_t s);
-int stencil_g(float* Gx, float* Gy, float* Gz, const float* phi, const float* gout, int w, int h,
-              int l, const rsf_params* p, hipStream_t s);
+// The forward step's normal n = g/(|g|+1e-6) and gradient g at every voxel, which
+// stencil_g computes and finish reads at a voxel's neighbours (its curvature and
+// Laplacian).
+struct FwdStencil {
+    float* n[3];
+    float* g[3];
+};
+int stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, const float* phi, const float* gout, int w,
+              int h, int l, const rsf_params* p, hipStream_t s);
 int finish_rows(int64_t n);
 int finish(float* gin, float* partials, const float* phi, const float* gout, const float* Gx,
-           const float* Gy, const float* Gz, const float* Bh, const float* Ba, const void* img, int u8,
-           const float* fo, const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s);
+           const float* Gy, const float* Gz, FwdStencil fwd, const float* Bh, const float* Ba, const void* img,
+           int u8, const float* fo, const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s);
 int reduce_params(float* gparams, const float* partials, int rows, hipStream_t s);
 }  // namespace adj
 
