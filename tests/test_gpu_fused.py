@@ -232,8 +232,8 @@ def _registered(mode, P=21):
     return s1, s2
 
 
-@pytest.mark.parametrize("shape", [(40, 44, 48), (52, 97, 130), (37, 131, 236)],
-                         ids=["40x44x48", "52x97x130", "37x131x236"])
+@pytest.mark.parametrize("shape", [(40, 44, 48), (52, 97, 130), (37, 131, 236), (29, 57, 301)],
+                         ids=["40x44x48", "52x97x130", "37x131x236", "29x57x301"])
 @pytest.mark.parametrize("lambdas", [(1.0, 1.0), (1.3, 0.7)])
 @pytest.mark.parametrize("dtype", [np.uint8, np.float32])
 @pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
@@ -249,8 +249,10 @@ def test_kernel_variants_give_identical_results(ctx, dtype, mode, lambdas, shape
     130), the first aligned rows (the vector sweep-2 interior); the third four
     64-column strips, the last partial, and a row count (131) that is not a
     multiple of the strip kernel's 21-row groups, with the strip kernel as the
-    literal default (wide enough for it).  In literal mode the default run is
-    also the oracle's phi bit for bit."""
+    literal default (wide enough for it); the fourth an odd width (rows that do
+    not start on 16-byte boundaries: the scalar sweep-2 interior) under the strips
+    and a row count below three ring groups.  In literal mode the default
+    run is also the oracle's phi bit for bit."""
     import torch
 
     img, phi0, _ = scene(shape, tubes=6, seed=23)
