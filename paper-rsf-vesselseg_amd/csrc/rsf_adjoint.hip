@@ -80,26 +80,87 @@ __device__ __forceinline__ float dT_yz(const U& u, int64_t j, int c, int n, int6
     return (plus - minus) * 0.5f;
 }
 
-__global__ void k_pointwise_q(float* QH, float* QA, const float* phi, const float* gout, const void* img, int u8,
-                              const float* fo, const float* fi, const float* PH, const float* PHc, int w, int h,
-                              int l, int m, float dt, float l1, float l2, float eps) {
+// The co-states of the blurred channels at voxel i: QH = PHbar - PHcbar and QA =
+// PAbar - PBbar inside the valid region, 0 in the pad band.
+struct QArgs {
+    const float *phi, *gout, *fo, *fi, *PH, *PHc;
+    const void* img;
+    int u8, m;
+    float dt, l1, l2, eps;
+};
+__device__ __forceinline__ void q_at(const QArgs& a, int64_t i, int x, int y, int z, int w, int h, int l, float& qh,
+                                     float& qa) {
+    const int m = a.m;
+    const bool valid = x >= m && x < w - m && y >= m && y < h - m && z >= m && z < l - m;
+    qh = 0.f;
+    qa = 0.f;
+    if (valid) {
+        const float I = image_at(a.img, a.u8, i);
+        const float Fb = -a.dt * dirac(a.phi[i], a.eps) * a.gout[i];
+        const float fob = -2.f * (I - a.fo[i]) * a.l1 * Fb;
+        const float fib = 2.f * (I - a.fi[i]) * a.l2 * Fb;
+        const float ph = a.PH[i], phc = a.PHc[i];
+        qh = -fob * a.fo[i] / ph + fib * a.fi[i] / phc;   // PHbar - PHcbar
+        qa = fob / ph - fib / phc;                        // PAbar - PBbar
+    }
+}
+
+__global__ void k_pointwise_q(float* QH, float* QA, QArgs a, int w, int h, int l) {
     const int64_t plane = (int64_t)w * h, n = plane * l;
     VOXELS(i, n) {
         int x, y, z;
         coords(i, w, h, n, x, y, z);
-        const bool valid = x >= m && x < w - m && y >= m && y < h - m && z >= m && z < l - m;
-        float qh = 0.f, qa = 0.f;
-        if (valid) {
-            const float I = image_at(img, u8, i);
-            const float Fb = -dt * dirac(phi[i], eps) * gout[i];
-            const float fob = -2.f * (I - fo[i]) * l1 * Fb;
-            const float fib = 2.f * (I - fi[i]) * l2 * Fb;
-            const float ph = PH[i], phc = PHc[i];
-            qh = -fob * fo[i] / ph + fib * fi[i] / phc;   // PHbar - PHcbar
-            qa = fob / ph - fib / phc;                    // PAbar - PBbar
-        }
+        float qh, qa;
+        q_at(a, i, x, y, z, w, h, l, qh, qa);
         QH[i] = qh;
         QA[i] = qa;
+    }
+}
+
+// The co-states and the x pass of their transposed blur in one kernel: a 64 x 4
+// thread block takes 4 rows at a time, forms QH and QA of the rows in LDS (2 x 4 x w
+// floats of dynamic LDS), and correlates them along x from there, every lane
+// running the same masked taps as k_conv_same_zero2 along x (the same bits as the
+// two kernels, without QH and QA through HBM).
+template <int P>
+__global__ __launch_bounds__(256) void k_q_convx(float* __restrict__ out0, float* __restrict__ out1, QArgs a, int w,
+                                                 int h, int l, Taps t) {
+    constexpr int m = (P - 1) / 2;
+    extern __shared__ float sq[];   // [4][2][w]
+    float* row = sq + threadIdx.y * 2 * w;
+    const int rows = h * l;
+    // every block runs the same number of iterations (the barriers): rows past the
+    // end compute nothing and store nothing
+    for (int r0 = blockIdx.x * 4; r0 < rows; r0 += gridDim.x * 4) {
+        const int r = r0 + threadIdx.y;
+        const bool live = r < rows;
+        const int y = live ? r % h : 0, z = live ? r / h : 0;
+        const int64_t base = (int64_t)r * w;
+        if (live) {
+            for (int x = threadIdx.x; x < w; x += 64) {
+                float qh, qa;
+                q_at(a, base + x, x, y, z, w, h, l, qh, qa);
+                row[x] = qh;
+                row[w + x] = qa;
+            }
+        }
+        __syncthreads();
+        if (live) {
+            for (int x = threadIdx.x; x < w; x += 64) {
+                float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    const int cc = x - m + k;
+                    const bool in_range = cc >= 0 && cc < w;
+                    const int q = min(max(cc, 0), w - 1);
+                    a0 = fmaf(t.g[k], in_range ? row[q] : 0.f, a0);
+                    a1 = fmaf(t.g[k], in_range ? row[w + q] : 0.f, a1);
+                }
+                out0[base + x] = a0;
+                out1[base + x] = a1;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -245,13 +306,11 @@ __global__ __launch_bounds__(256) void k_conv_same_zero2r(float* __restrict__ ou
     }
 }
 
+// the y and z passes: Bh, Ba (the x pass) -> t1, t2 -> Bh, Ba
 template <int P>
-int conv2_axes(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w, int h, int l,
-               const Taps& taps, hipStream_t s) {
+int conv2_yz(float* Bh, float* Ba, float* t1, float* t2, int w, int h, int l, const Taps& taps, hipStream_t s) {
     constexpr int R = 8;
     const dim3 blk(64, 4);
-    const dim3 grid((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)h * l + 3) / 4, 65535));
-    hipLaunchKernelGGL((k_conv_same_zero2<0, P>), grid, blk, 0, s, Bh, Ba, QH, QA, w, h, l, taps);
     const dim3 gy((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)((h + R - 1) / R) * l + 3) / 4, 65535));
     hipLaunchKernelGGL((k_conv_same_zero2r<1, P, R>), gy, blk, 0, s, t1, t2, (const float*)Bh, (const float*)Ba, w, h,
                        l, taps);
@@ -260,6 +319,25 @@ int conv2_axes(float* Bh, float* Ba, const float* QH, const float* QA, float* t1
                        l, taps);
     RSF_HIP(hipGetLastError());
     return RSF_OK;
+}
+
+template <int P>
+int conv2_axes(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w, int h, int l,
+               const Taps& taps, hipStream_t s) {
+    const dim3 blk(64, 4);
+    const dim3 grid((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)h * l + 3) / 4, 65535));
+    hipLaunchKernelGGL((k_conv_same_zero2<0, P>), grid, blk, 0, s, Bh, Ba, QH, QA, w, h, l, taps);
+    return conv2_yz<P>(Bh, Ba, t1, t2, w, h, l, taps, s);
+}
+
+// the co-states formed inside the x pass (k_q_convx), then the y and z passes
+template <int P>
+int q_conv2_axes(float* Bh, float* Ba, const QArgs& a, float* t1, float* t2, int w, int h, int l, const Taps& taps,
+                 hipStream_t s) {
+    const dim3 blk(64, 4);
+    const dim3 grid((unsigned)std::min<int64_t>(((int64_t)h * l + 3) / 4, 65535));
+    hipLaunchKernelGGL((k_q_convx<P>), grid, blk, (size_t)8 * w * sizeof(float), s, Bh, Ba, a, w, h, l, taps);
+    return conv2_yz<P>(Bh, Ba, t1, t2, w, h, l, taps, s);
 }
 
 // G_a at every voxel: normalisation Jacobian applied to D^T(kbar) plus D^T(Lbar).
@@ -385,8 +463,8 @@ int pointwise_q(float* QH, float* QA, const float* phi, const float* gout, const
                 const float* fi, const float* PH, const float* PHc, int w, int h, int l, int m, const rsf_params* p,
                 hipStream_t s) {
     const int64_t n = (int64_t)w * h * l;
-    hipLaunchKernelGGL(k_pointwise_q, grid_for(n), dim3(kT), 0, s, QH, QA, phi, gout, img, u8, fo, fi, PH, PHc, w, h,
-                       l, m, p->dt, p->lambda1, p->lambda2, p->epsilon);
+    const QArgs a{phi, gout, fo, fi, PH, PHc, img, u8, m, p->dt, p->lambda1, p->lambda2, p->epsilon};
+    hipLaunchKernelGGL(k_pointwise_q, grid_for(n), dim3(kT), 0, s, QH, QA, a, w, h, l);
     RSF_HIP(hipGetLastError());
     return RSF_OK;
 }
@@ -401,6 +479,23 @@ int blur_transpose(float* out, const float* in, float* t1, float* t2, int w, int
                        d.m);
     RSF_HIP(hipGetLastError());
     return RSF_OK;
+}
+
+int q_blur_transpose2(float* Bh, float* Ba, float* QH, float* QA, float* t1, float* t2, const float* phi,
+                      const float* gout, const void* img, int u8, const float* fo, const float* fi, const float* PH,
+                      const float* PHc, int w, int h, int l, const Derived& d, const rsf_params* p, hipStream_t s) {
+    const QArgs a{phi, gout, fo, fi, PH, PHc, img, u8, d.m, p->dt, p->lambda1, p->lambda2, p->epsilon};
+    // rows of up to 2048 voxels fit the fused kernel's 64 KB of LDS
+    if (w <= 2048) {
+        switch (d.P) {
+            case 7: return q_conv2_axes<7>(Bh, Ba, a, t1, t2, w, h, l, d.taps, s);
+            case 21: return q_conv2_axes<21>(Bh, Ba, a, t1, t2, w, h, l, d.taps, s);
+            case 35: return q_conv2_axes<35>(Bh, Ba, a, t1, t2, w, h, l, d.taps, s);
+            default: break;
+        }
+    }
+    RSF_TRY(pointwise_q(QH, QA, phi, gout, img, u8, fo, fi, PH, PHc, w, h, l, d.m, p, s));
+    return blur_transpose2(Bh, Ba, QH, QA, t1, t2, w, h, l, d, s);
 }
 
 int blur_transpose2(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w, int h,

@@ -1302,9 +1302,8 @@ int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks
     for (auto& b : pr.blocks) {
         const int64_t o = b.off;
         const void* im = (const char*)image + o * esz;
-        RSF_TRY(adj::pointwise_q(QH + o, QA + o, phi + o, grad_out + o, im, u8, fo + o, fi + o, PH + o, PHc + o, b.w,
-                                 b.h, b.l, pr.d.m, p, s));
-        RSF_TRY(adj::blur_transpose2(Bh + o, Ba + o, QH + o, QA + o, t1 + o, t2 + o, b.w, b.h, b.l, pr.d, s));
+        RSF_TRY(adj::q_blur_transpose2(Bh + o, Ba + o, QH + o, QA + o, t1 + o, t2 + o, phi + o, grad_out + o, im, u8,
+                                       fo + o, fi + o, PH + o, PHc + o, b.w, b.h, b.l, pr.d, p, s));
         // the forward normals and gradients into the buffers free by now (PH, PHc:
         // read by the point-wise step; QH, QA, t1, t2: by the transposed blur)
         const adj::FwdStencil fwd{{PH + o, PHc + o, QH + o}, {QA + o, t1 + o, t2 + o}};

@@ -191,6 +191,11 @@ int blur_transpose(float* out, const float* in, float* t1, float* t2, int w, int
 // Both co-states at once (Bh <- blur^T QH, Ba <- blur^T QA); t1, t2 are scratch.
 int blur_transpose2(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w,
                     int h, int l, const Derived& d, hipStream_t s);
+// pointwise_q then blur_transpose2, the co-states formed inside the x pass where
+// the tap count and the row length allow (QH, QA then untouched)
+int q_blur_transpose2(float* Bh, float* Ba, float* QH, float* QA, float* t1, float* t2, const float* phi,
+                      const float* gout, const void* img, int u8, const float* fo, const float* fi, const float* PH,
+                      const float* PHc, int w, int h, int l, const Derived& d, const rsf_params* p, hipStream_t s);
 // The forward step's normal n = g/(|g|+1e-6) and gradient g at every voxel, which
 // stencil_g computes and finish reads at a voxel's neighbours (its curvature and
 // Laplacian).
