@@ -801,7 +801,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
     constexpr int NV = (NWIN + VW - 1) / VW;
     constexpr int SEP = 64 * NE;              // staged row length (every lane stages NE elements)
     static_assert(NCOL % VW == 0 && (SW <= 64 || !LIT) && (NCG - 1) * NCOL + NV * VW <= SEP, "strip shape");
-    static_assert(P % D == 0 && NCOL <= M, "P must be a multiple of the load queue depth");
+    static_assert(P % D == 0 && NCOL <= M, "P a multiple of the load queue depth, NCOL at most M");
     typedef float fvec __attribute__((ext_vector_type(NCOL), aligned(4)));   // a lane's columns (fast mode)
     const int img_u8 = IMG >= 0 ? IMG : img_u8_arg;
     __shared__ __align__(16) float sRow[NW][CH][SEP];
