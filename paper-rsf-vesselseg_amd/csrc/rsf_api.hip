@@ -274,6 +274,21 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
                    bool aligned, int* s1, int* s2, int* zc1, int* zc2) {
     *s1 = default_sweep1(d.P, literal);
     *s2 = default_sweep2(literal);
+    // a z kernel of wide work items (1024 columns, one workgroup per CU) is for
+    // planes that give every CU one: smaller planes take the first variant whose z
+    // work items are 256 columns
+    if (*s1 >= 0 && sweep1_variants()[*s1].fn2) {
+        const int64_t span = (int64_t)sweep1_variants()[*s1].tx2 * sweep1_variants()[*s1].ty2;
+        bool small_plane = false;
+        for (const auto& b : blocks) small_plane = small_plane || (int64_t)b.w * b.h < span * c->num_cus;
+        for (size_t k = 0; span > 256 && small_plane && k < sweep1_variants().size(); ++k) {
+            const auto& v = sweep1_variants()[k];
+            if (v.P == d.P && v.literal == literal && v.fn2 && (int64_t)v.tx2 * v.ty2 <= 256) {
+                *s1 = (int)k;
+                break;
+            }
+        }
+    }
     // a strip xy kernel wastes the lanes past a block's valid columns: blocks
     // narrower than three strips take the first tiled variant
     if (*s1 >= 0 && sweep1_variants()[*s1].strip > 0) {

@@ -2517,7 +2517,12 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1strip<21, false, 2, 4, 3, 4, 256, 3>("p21_fast_strip_64w4"),
         // literal default: the strip xy kernel, 64-column strips, a plane per wave (xy
         // kernel at 512^3: 0.82 ms against 1.02 for the 32x60 tiles below), with the
-        // interleaved z kernel
+        // interleaved z kernel on work items of 1024 consecutive columns, one 1024-thread
+        // workgroup per CU (0.520 ms against 0.543 with 256 columns at 512^3: a
+        // workgroup's reads of a plane span 16 KB instead of 4)
+        s1strip<21, true, 4, 4, 1, 4, 1024, 3>("p21_lit_strip_64z1k"),
+        // the same with 256-column z work items: the literal default for planes that do
+        // not give every CU a 1024-column work item
         s1strip<21, true, 4, 4, 1, 4, 256, 3>("p21_lit_strip_64w4"),
         // the literal default for blocks narrower than three strips (64 uneven blocks of
         // sides 96 to 256: 3% faster than the strips): 32x60 tiles, 15 outputs per y

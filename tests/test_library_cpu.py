@@ -229,9 +229,9 @@ def test_variant_query_names_the_tuner_candidates():
     v1, v2 = rsf_amd.variants(1), rsf_amd.variants(2)
     names1 = {n for n, _, _ in v1}
     assert {"p21_lit_hi_32x44x8", "p21_lit_hi_32x60r15", "p21_fast_split_64x44w4", "p21_lit_strip_64w4",
-            "p21_lit_strip_64w3"} <= names1
+            "p21_lit_strip_64w3", "p21_lit_strip_64z1k", "p21_fast_strip_128w4"} <= names1
     # the defaults (the first P = 21 entry of each mode) are the strip kernels
-    assert next(n for n, t, l in v1 if t == 21 and l) == "p21_lit_strip_64w4"
+    assert next(n for n, t, l in v1 if t == 21 and l) == "p21_lit_strip_64z1k"
     assert next(n for n, t, l in v1 if t == 21 and not l) == "p21_fast_strip_128w4"
     for P in range(7, 37, 2):
         for lit in (False, True):
