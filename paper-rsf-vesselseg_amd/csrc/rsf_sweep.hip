@@ -805,7 +805,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
     typedef float fvec __attribute__((ext_vector_type(NCOL), aligned(4)));   // a lane's columns (fast mode)
     const int img_u8 = IMG >= 0 ? IMG : img_u8_arg;
     __shared__ __align__(16) float sRow[NW][CH][SEP];
-    __shared__ __align__(16) float sT[LIT ? NW : 1][LIT ? SW * CH : 1];
+    // the transposition: voxel v's channels at 4 v + 4 (v / 16), so that the writes of
+    // one column offset o by all 64 lanes (voxels 4 g + o, 4 channels) fall in 64
+    // distinct banks (unpadded, lanes g and g + 4 collide)
+    __shared__ __align__(16) float sT[LIT ? NW : 1][LIT ? SW * CH + 4 * (SW / 16) : 1];
+    auto tpos = [](int v) { return 4 * v + 4 * (v >> 4); };
     // the wave index in an SGPR: the planes, and with them the buffer resources, are wave-uniform
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int ch = lane / NCG, cg = lane % NCG;
@@ -930,9 +934,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
                         const uint32_t yo = (uint32_t)min(t - M, h - 1);
                         if constexpr (LIT) {
 #pragma unroll
-                            for (int o = 0; o < NCOL; ++o) sT[wv][(cg * NCOL + o) * CH + ch] = acc[o][js];
+                            for (int o = 0; o < NCOL; ++o) sT[wv][tpos(cg * NCOL + o) + ch] = acc[o][js];
                             __builtin_amdgcn_wave_barrier();
-                            const float4 vv = *reinterpret_cast<const float4*>(&sT[wv][min(lane, SW - 1) * CH]);
+                            const float4 vv = *reinterpret_cast<const float4*>(&sT[wv][tpos(min(lane, SW - 1))]);
                             const u32x4 b = {__float_as_uint(vv.x), __float_as_uint(vv.y), __float_as_uint(vv.z),
                                              __float_as_uint(vv.w)};
                             __builtin_amdgcn_raw_buffer_store_b128(b, rX, 16 * gxo, (int)(16u * yo * (uint32_t)w),
