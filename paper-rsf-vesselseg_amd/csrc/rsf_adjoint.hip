@@ -16,6 +16,8 @@
 // valid-region adjoint with zeros outside.  D^T follows the reference edge
 // rules: along x the left operand is the previous linear element (wrap), so
 // its transpose gathers from the next linear element.
+#include <initializer_list>
+
 #include "rsf_internal.hpp"
 
 namespace rsf {
@@ -50,9 +52,11 @@ __device__ __forceinline__ void coords(int64_t i, int w, int h, int64_t n, int& 
 }
 
 __device__ __forceinline__ float dirac(float p, float eps) {
+#pragma clang fp contract(off)
     return (float)(1.0 / kPi) * eps / (eps * eps + p * p);
 }
 __device__ __forceinline__ float dirac_prime(float p, float eps) {
+#pragma clang fp contract(off)
     const float q = eps * eps + p * p;
     return -(float)(2.0 / kPi) * eps * p / (q * q);
 }
@@ -63,6 +67,7 @@ __device__ __forceinline__ float image_at(const void* img, int u8, int64_t i) {
 // (D_a^T u)(j) for a value getter u(q) over block-local linear indices.
 template <class U>
 __device__ __forceinline__ float dT_x(const U& u, int64_t j, int x, int w, int64_t N) {
+#pragma clang fp contract(off)
     float plus = 0.f, minus = 0.f;
     if (x != 0) plus += u(j - 1);
     if (x == w - 1) plus += u(j);
@@ -72,6 +77,7 @@ __device__ __forceinline__ float dT_x(const U& u, int64_t j, int x, int w, int64
 }
 template <class U>
 __device__ __forceinline__ float dT_yz(const U& u, int64_t j, int c, int n, int64_t stride) {
+#pragma clang fp contract(off)
     float plus = 0.f, minus = 0.f;
     if (c >= 1) plus += u(j - stride);
     if (c == n - 1) plus += u(j);
@@ -344,6 +350,7 @@ int q_conv2_axes(float* Bh, float* Ba, const QArgs& a, float* t1, float* t2, int
 // Also the forward normal and gradient at the voxel, for k_finish.
 __global__ void k_stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, const float* phi, const float* gout,
                             int w, int h, int l, float dt, float nu, float mu, float eps) {
+#pragma clang fp contract(off)
     const int64_t plane = (int64_t)w * h, n = plane * l;
     auto kbar = [&](int64_t q) { return (nu * dt * dirac(phi[q], eps) - mu * dt) * gout[q]; };
     auto lbar = [&](int64_t q) { return mu * dt * gout[q]; };
@@ -368,6 +375,32 @@ __global__ void k_stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, con
     }
 }
 
+// Per-workgroup partial sums in a fixed order: a shuffle tree within each 64-lane
+// wave, then the waves; row blockIdx.x of partials.
+__device__ __forceinline__ void block_partials(float* __restrict__ partials, float s0, float s1, float s2, float s3) {
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_down(s0, o, 64);
+        s1 += __shfl_down(s1, o, 64);
+        s2 += __shfl_down(s2, o, 64);
+        s3 += __shfl_down(s3, o, 64);
+    }
+    constexpr int kW = kT / 64;
+    __shared__ float red[4][kW];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wv] = s0;
+        red[1][wv] = s1;
+        red[2][wv] = s2;
+        red[3][wv] = s3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        float v = red[threadIdx.x][0];
+        for (int q = 1; q < kW; ++q) v += red[threadIdx.x][q];
+        partials[4 * blockIdx.x + threadIdx.x] = v;
+    }
+}
+
 // dphi and per-workgroup partial sums of the parameter gradients.
 __global__ __launch_bounds__(kT) void k_finish(float* gin, float* partials, const float* phi, const float* gout,
                                                const float* Gx, const float* Gy, const float* Gz, FwdStencil fwd,
@@ -375,6 +408,7 @@ __global__ __launch_bounds__(kT) void k_finish(float* gin, float* partials, cons
                                                const float* Ba, const void* img, int u8, const float* fo,
                                                const float* fi, int w, int h, int l, float dt, float nu, float mu,
                                                float eps, float l1, float l2) {
+#pragma clang fp contract(off)
     const int64_t plane = (int64_t)w * h, n = plane * l;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
     auto gx = [&](int64_t q) { return Gx[q]; };
@@ -408,28 +442,216 @@ __global__ __launch_bounds__(kT) void k_finish(float* gin, float* partials, cons
         s2 += dt * d * kap * u;
         s3 += dt * (lap - kap) * u;
     }
-    // fixed-order reduction: a shuffle tree within each 64-lane wave, then the waves
-    for (int o = 32; o > 0; o >>= 1) {
-        s0 += __shfl_down(s0, o, 64);
-        s1 += __shfl_down(s1, o, 64);
-        s2 += __shfl_down(s2, o, 64);
-        s3 += __shfl_down(s3, o, 64);
+    block_partials(partials, s0, s1, s2, s3);
+}
+
+// k_stencil_g and k_finish on four consecutive x voxels per thread: the same
+// arithmetic per voxel, on 16-byte loads and stores and 32-bit indices, the x
+// neighbours shared within the quad.  For rows a multiple of 4 long, blocks under
+// 2^31 voxels and arrays 16-byte aligned (a uint8 image 4-byte aligned).  The
+// neighbour rows and planes are read clamped to the quad's own at the block's
+// faces, where the one-sided rules take the voxel itself or drop the term.  Both
+// kernel pairs compile without contraction, so that no multiply-add is fused in
+// one and not the other (which the compiler decides by how values are shared): a
+// block gives the same bits on either.
+__device__ __forceinline__ void ld4(float (&a)[4], const float* __restrict__ p, uint32_t i) {
+    const float4 v = *reinterpret_cast<const float4*>(p + i);
+    a[0] = v.x;
+    a[1] = v.y;
+    a[2] = v.z;
+    a[3] = v.w;
+}
+__device__ __forceinline__ void st4(float* __restrict__ p, uint32_t i, const float (&a)[4]) {
+    *reinterpret_cast<float4*>(p + i) = make_float4(a[0], a[1], a[2], a[3]);
+}
+// a row's values at x0-1 .. x0+4: the previous linear element (element 0 for the
+// block's first voxel), the quad, the next linear element (clamped at the end)
+__device__ __forceinline__ void ld6(float (&a)[6], const float* __restrict__ p, uint32_t base, uint32_t n) {
+    a[0] = p[base == 0 ? 0 : base - 1];
+    a[5] = p[base + 4 < n ? base + 4 : n - 1];
+    float c[4];
+    ld4(c, p, base);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i + 1] = c[i];
+}
+// dT_x and dT_yz on values: uL = u(j-1), u0 = u(j), uR = u(j+1) (uZ = u(0) at j = 0);
+// um, up the neighbours along y or z
+__device__ __forceinline__ float dTx_v(float uL, float u0, float uR, uint32_t j, int x, int w, uint32_t n) {
+#pragma clang fp contract(off)
+    float plus = 0.f, minus = 0.f;
+    if (x != 0) plus += uL;
+    if (x == w - 1) plus += u0;
+    if (j + 1 < n) minus += uR;
+    if (j == 0) minus += u0;
+    return (plus - minus) * 0.5f;
+}
+__device__ __forceinline__ float dTyz_v(float um, float u0, float up, int c, int nn) {
+#pragma clang fp contract(off)
+    float plus = 0.f, minus = 0.f;
+    if (c >= 1) plus += um;
+    if (c == nn - 1) plus += u0;
+    if (c == 0) minus += u0;
+    if (c <= nn - 2) minus += up;
+    return (plus - minus) * 0.5f;
+}
+__device__ __forceinline__ void quad_coords(uint32_t base, int w, int h, int& x0, int& y, int& z) {
+    const uint32_t r = base / (uint32_t)w;
+    x0 = (int)(base - r * (uint32_t)w);
+    z = (int)(r / (uint32_t)h);
+    y = (int)(r - (uint32_t)z * (uint32_t)h);
+}
+
+__global__ __launch_bounds__(kT) void k_stencil_g4(float* __restrict__ Gx, float* __restrict__ Gy,
+                                                   float* __restrict__ Gz, FwdStencil fwd,
+                                                   const float* __restrict__ phi, const float* __restrict__ gout,
+                                                   int w, int h, int l, float dt, float nu, float mu, float eps) {
+#pragma clang fp contract(off)
+    const uint32_t W = (uint32_t)w, plane = W * (uint32_t)h, n = plane * (uint32_t)l;
+    auto kbar = [&](float p, float u) { return (nu * dt * dirac(p, eps) - mu * dt) * u; };
+    auto lbar = [&](float u) { return mu * dt * u; };
+    for (uint32_t k = blockIdx.x * kT + threadIdx.x; k < n / 4; k += gridDim.x * kT) {
+        const uint32_t base = 4 * k;
+        int x0, y, z;
+        quad_coords(base, w, h, x0, y, z);
+        const uint32_t iym = y > 0 ? base - W : base, iyp = y + 1 < h ? base + W : base;
+        const uint32_t izm = z > 0 ? base - plane : base, izp = z + 1 < l ? base + plane : base;
+        float pr[6], ur[6], pym[4], pyp[4], pzm[4], pzp[4], uym[4], uyp[4], uzm[4], uzp[4];
+        ld6(pr, phi, base, n);
+        ld6(ur, gout, base, n);
+        ld4(pym, phi, iym);
+        ld4(pyp, phi, iyp);
+        ld4(pzm, phi, izm);
+        ld4(pzp, phi, izp);
+        ld4(uym, gout, iym);
+        ld4(uyp, gout, iyp);
+        ld4(uzm, gout, izm);
+        ld4(uzp, gout, izp);
+        float kr[6], lr[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            kr[t] = kbar(pr[t], ur[t]);
+            lr[t] = lbar(ur[t]);
+        }
+        float oGx[4], oGy[4], oGz[4], n0[4], n1[4], n2[4], g0[4], g1[4], g2[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t j = base + i;
+            const int x = x0 + i;
+            const float nbx = dTx_v(kr[i], kr[i + 1], kr[i + 2], j, x, w, n);
+            const float nby = dTyz_v(kbar(pym[i], uym[i]), kr[i + 1], kbar(pyp[i], uyp[i]), y, h);
+            const float nbz = dTyz_v(kbar(pzm[i], uzm[i]), kr[i + 1], kbar(pzp[i], uzp[i]), z, l);
+            const float lbx = dTx_v(lr[i], lr[i + 1], lr[i + 2], j, x, w, n);
+            const float lby = dTyz_v(lbar(uym[i]), lr[i + 1], lbar(uyp[i]), y, h);
+            const float lbz = dTyz_v(lbar(uzm[i]), lr[i + 1], lbar(uzp[i]), z, l);
+            // grad_at: right x neighbour the voxel itself at x = w-1, left the previous
+            // linear element; y and z from the clamped rows and planes
+            Grad g;
+            g.x = ((x == w - 1 ? pr[i + 1] : pr[i + 2]) - pr[i]) * 0.5f;
+            g.y = (pyp[i] - pym[i]) * 0.5f;
+            g.z = (pzp[i] - pzm[i]) * 0.5f;
+            const float M = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+            const float r = 1.0f / (M + kEta);
+            const float c = M > 0.f ? (nbx * g.x + nby * g.y + nbz * g.z) * r * r / M : 0.f;
+            oGx[i] = nbx * r - g.x * c + lbx;
+            oGy[i] = nby * r - g.y * c + lby;
+            oGz[i] = nbz * r - g.z * c + lbz;
+            n0[i] = g.x / (M + kEta);
+            n1[i] = g.y / (M + kEta);
+            n2[i] = g.z / (M + kEta);
+            g0[i] = g.x;
+            g1[i] = g.y;
+            g2[i] = g.z;
+        }
+        st4(Gx, base, oGx);
+        st4(Gy, base, oGy);
+        st4(Gz, base, oGz);
+        st4(fwd.n[0], base, n0);
+        st4(fwd.n[1], base, n1);
+        st4(fwd.n[2], base, n2);
+        st4(fwd.g[0], base, g0);
+        st4(fwd.g[1], base, g1);
+        st4(fwd.g[2], base, g2);
     }
-    constexpr int kW = kT / 64;
-    __shared__ float red[4][kW];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) {
-        red[0][wv] = s0;
-        red[1][wv] = s1;
-        red[2][wv] = s2;
-        red[3][wv] = s3;
+}
+
+__global__ __launch_bounds__(kT) void k_finish4(float* __restrict__ gin, float* __restrict__ partials,
+                                                const float* __restrict__ phi, const float* __restrict__ gout,
+                                                const float* __restrict__ Gx, const float* __restrict__ Gy,
+                                                const float* __restrict__ Gz, FwdStencil fwd,
+                                                const float* __restrict__ Bh, const float* __restrict__ Ba,
+                                                const void* __restrict__ img, int u8, const float* __restrict__ fo,
+                                                const float* __restrict__ fi, int w, int h, int l, float dt, float nu,
+                                                float mu, float eps, float l1, float l2) {
+#pragma clang fp contract(off)
+    const uint32_t W = (uint32_t)w, plane = W * (uint32_t)h, n = plane * (uint32_t)l;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    for (uint32_t k = blockIdx.x * kT + threadIdx.x; k < n / 4; k += gridDim.x * kT) {
+        const uint32_t base = 4 * k;
+        int x0, y, z;
+        quad_coords(base, w, h, x0, y, z);
+        const uint32_t iym = y > 0 ? base - W : base, iyp = y + 1 < h ? base + W : base;
+        const uint32_t izm = z > 0 ? base - plane : base, izp = z + 1 < l ? base + plane : base;
+        // G along x, y, z; the forward normals and gradients: component x along the
+        // row, y at the rows y+-1, z at the planes z+-1
+        float gxr[6], n0r[6], g0r[6], gy0[4], gym[4], gyp[4], gz0[4], gzm[4], gzp[4];
+        float n1m[4], n1p[4], g1m[4], g1p[4], n2m[4], n2p[4], g2m[4], g2p[4];
+        ld6(gxr, Gx, base, n);
+        ld6(n0r, fwd.n[0], base, n);
+        ld6(g0r, fwd.g[0], base, n);
+        ld4(gy0, Gy, base);
+        ld4(gym, Gy, iym);
+        ld4(gyp, Gy, iyp);
+        ld4(gz0, Gz, base);
+        ld4(gzm, Gz, izm);
+        ld4(gzp, Gz, izp);
+        ld4(n1m, fwd.n[1], iym);
+        ld4(n1p, fwd.n[1], iyp);
+        ld4(g1m, fwd.g[1], iym);
+        ld4(g1p, fwd.g[1], iyp);
+        ld4(n2m, fwd.n[2], izm);
+        ld4(n2p, fwd.n[2], izp);
+        ld4(g2m, fwd.g[2], izm);
+        ld4(g2p, fwd.g[2], izp);
+        float I[4], fo4[4], fi4[4], p4[4], u4[4], bh[4], ba[4];
+        if (u8) {
+            const uint32_t v = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(img) + base);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) I[i] = (float)((v >> (8 * i)) & 255u);
+        } else {
+            ld4(I, static_cast<const float*>(img), base);
+        }
+        ld4(fo4, fo, base);
+        ld4(fi4, fi, base);
+        ld4(p4, phi, base);
+        ld4(u4, gout, base);
+        ld4(bh, Bh, base);
+        ld4(ba, Ba, base);
+        float o[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t j = base + i;
+            const int x = x0 + i;
+            const float st = dTx_v(gxr[i], gxr[i + 1], gxr[i + 2], j, x, w, n) + dTyz_v(gym[i], gy0[i], gyp[i], y, h) +
+                             dTyz_v(gzm[i], gz0[i], gzp[i], z, l);
+            const float kap = (((x == w - 1 ? n0r[i + 1] : n0r[i + 2]) - n0r[i]) + (n1p[i] - n1m[i]) +
+                               (n2p[i] - n2m[i])) * 0.5f;
+            const float lap = (((x == w - 1 ? g0r[i + 1] : g0r[i + 2]) - g0r[i]) + (g1p[i] - g1m[i]) +
+                               (g2p[i] - g2m[i])) * 0.5f;
+            const float d1 = I[i] - fo4[i], d2 = I[i] - fi4[i];
+            const float Eo = d1 * d1, Ei = d2 * d2;
+            const float F = l1 * Eo - l2 * Ei;
+            const float p0 = p4[i], u = u4[i];
+            const float d = dirac(p0, eps);
+            const float Fb = -dt * d * u;
+            o[i] = u + dirac_prime(p0, eps) * (-dt * F + nu * dt * kap) * u + d * (bh[i] + I[i] * ba[i]) + st;
+            s0 += Eo * Fb;
+            s1 -= Ei * Fb;
+            s2 += dt * d * kap * u;
+            s3 += dt * (lap - kap) * u;
+        }
+        st4(gin, base, o);
     }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        float v = red[threadIdx.x][0];
-        for (int q = 1; q < kW; ++q) v += red[threadIdx.x][q];
-        partials[4 * blockIdx.x + threadIdx.x] = v;
-    }
+    block_partials(partials, s0, s1, s2, s3);
 }
 
 // grad_params[q] += sum over the partial rows, in double.  One workgroup: thread
@@ -510,11 +732,25 @@ int blur_transpose2(float* Bh, float* Ba, const float* QH, const float* QA, floa
     }
 }
 
+// the quad kernels' conditions: rows a multiple of 4 long, under 2^31 voxels, every
+// pointer 16-byte aligned
+bool quads_fit(int w, int64_t n, std::initializer_list<const void*> ptrs) {
+    if (w % 4 != 0 || n >= (int64_t(1) << 31)) return false;
+    for (const void* q : ptrs)
+        if (reinterpret_cast<uintptr_t>(q) % 16 != 0) return false;
+    return true;
+}
+
 int stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, const float* phi, const float* gout, int w, int h,
               int l, const rsf_params* p, hipStream_t s) {
     const int64_t n = (int64_t)w * h * l;
-    hipLaunchKernelGGL(k_stencil_g, grid_for(n), dim3(kT), 0, s, Gx, Gy, Gz, fwd, phi, gout, w, h, l, p->dt, p->nu,
-                       p->mu, p->epsilon);
+    if (quads_fit(w, n, {Gx, Gy, Gz, fwd.n[0], fwd.n[1], fwd.n[2], fwd.g[0], fwd.g[1], fwd.g[2], phi, gout})) {
+        hipLaunchKernelGGL(k_stencil_g4, grid_for(n / 4), dim3(kT), 0, s, Gx, Gy, Gz, fwd, phi, gout, w, h, l, p->dt,
+                           p->nu, p->mu, p->epsilon);
+    } else {
+        hipLaunchKernelGGL(k_stencil_g, grid_for(n), dim3(kT), 0, s, Gx, Gy, Gz, fwd, phi, gout, w, h, l, p->dt,
+                           p->nu, p->mu, p->epsilon);
+    }
     RSF_HIP(hipGetLastError());
     return RSF_OK;
 }
@@ -530,9 +766,16 @@ int finish(float* gin, float* partials, const float* phi, const float* gout, con
            const float* Gz, FwdStencil fwd, const float* Bh, const float* Ba, const void* img, int u8,
            const float* fo, const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s) {
     const int64_t n = (int64_t)w * h * l;
-    hipLaunchKernelGGL(k_finish, grid_finish(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, fwd, Bh, Ba,
-                       img, u8,
-                       fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2);
+    // the same grid either way (finish_rows partial rows); a uint8 image is read 4 bytes at a time
+    const bool img_ok = reinterpret_cast<uintptr_t>(img) % (u8 ? 4 : 16) == 0;
+    if (img_ok && quads_fit(w, n, {gin, phi, gout, Gx, Gy, Gz, fwd.n[0], fwd.n[1], fwd.n[2], fwd.g[0], fwd.g[1],
+                                   fwd.g[2], Bh, Ba, fo, fi})) {
+        hipLaunchKernelGGL(k_finish4, grid_finish(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, fwd, Bh,
+                           Ba, img, u8, fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2);
+    } else {
+        hipLaunchKernelGGL(k_finish, grid_finish(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, fwd, Bh,
+                           Ba, img, u8, fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2);
+    }
     RSF_HIP(hipGetLastError());
     return RSF_OK;
 }

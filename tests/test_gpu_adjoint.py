@@ -79,6 +79,35 @@ def test_adjoint_blocks_equal_single(ctx):
     np.testing.assert_allclose(gp.cpu().numpy(), gp_sum, rtol=1e-5)
 
 
+@pytest.mark.parametrize("shape,u8", [((36, 40, 44), True), ((32, 30, 48), False), ((40, 36, 128), True)])
+def test_adjoint_quad_kernels_equal_scalar(ctx, shape, u8):
+    """Rows a multiple of 4 long on 16-byte aligned buffers run the stencil and
+    finishing kernels four voxels per thread; the same block one voxel further on
+    runs them a voxel per thread.  dphi must be the same bits, the parameter
+    gradients the same up to the order of their partial sums."""
+    import torch
+
+    img, phi0, _ = scene(shape, tubes=5, seed=23)
+    n = img.size
+    u = np.random.default_rng(8).normal(size=n).astype(np.float32)
+    p = _params(3.0, lambda1=1.2, lambda2=0.9)
+    l, h, w = shape
+    out = []
+    for shift in (0, 1):
+        def pad(a):
+            b = np.zeros(n + shift, a.dtype)
+            b[shift:] = a.ravel()
+            return b
+        d_img = to_dev(pad(img) if u8 else pad(img.astype(np.float32)))
+        gin = torch.zeros(n + shift, device="cuda")
+        gp = torch.zeros(4, device="cuda")
+        ctx.adjoint([(shift, w, h, l)], d_img, to_dev(pad(phi0)), to_dev(pad(u)), gin, gp, p)
+        out.append((gin[shift:].cpu().numpy(), gp.cpu().numpy()))
+    (g0, p0), (g1, p1) = out
+    assert np.array_equal(g0.view(np.uint32), g1.view(np.uint32))
+    np.testing.assert_allclose(p0, p1, rtol=1e-5)
+
+
 def _fwd64(phi, img, p, K):
     st = [phi.astype(np.float64)]
     for _ in range(K):
