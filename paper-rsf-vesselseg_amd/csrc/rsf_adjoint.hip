@@ -111,6 +111,17 @@ __device__ __forceinline__ void q_at(const QArgs& a, int64_t i, int x, int y, in
     }
 }
 
+__device__ __forceinline__ QArgs shift_q(QArgs a, int64_t o) {
+    a.phi += o;
+    a.gout += o;
+    a.fo += o;
+    a.fi += o;
+    a.PH += o;
+    a.PHc += o;
+    a.img = static_cast<const char*>(a.img) + o * (a.u8 ? 1 : 4);
+    return a;
+}
+
 __global__ void k_pointwise_q(float* QH, float* QA, QArgs a, int w, int h, int l) {
     const int64_t plane = (int64_t)w * h, n = plane * l;
     VOXELS(i, n) {
@@ -127,11 +138,18 @@ __global__ void k_pointwise_q(float* QH, float* QA, QArgs a, int w, int h, int l
 // thread block takes 4 rows at a time, forms QH and QA of the rows in LDS (2 x 4 x w
 // floats of dynamic LDS), and correlates them along x from there, every lane
 // running the same masked taps as k_conv_same_zero2 along x (the same bits as the
-// two kernels, without QH and QA through HBM).
+// two kernels, without QH and QA through HBM).  With a block table bt, blockIdx.y
+// is the block (every block of the launch w x h x l), its first voxel bt[.].off.
 template <int P>
 __global__ __launch_bounds__(256) void k_q_convx(float* __restrict__ out0, float* __restrict__ out1, QArgs a, int w,
-                                                 int h, int l, Taps t) {
+                                                 int h, int l, Taps t, const BlockDesc* __restrict__ bt) {
     constexpr int m = (P - 1) / 2;
+    if (bt) {
+        const int64_t o = bt[blockIdx.y].off;
+        out0 += o;
+        out1 += o;
+        a = shift_q(a, o);
+    }
     extern __shared__ float sq[];   // [4][2][w]
     float* row = sq + threadIdx.y * 2 * w;
     const int rows = h * l;
@@ -253,8 +271,15 @@ template <int AXIS, int P, int R>
 __global__ __launch_bounds__(256) void k_conv_same_zero2r(float* __restrict__ out0, float* __restrict__ out1,
                                                           const float* __restrict__ in0,
                                                           const float* __restrict__ in1, int w, int h, int l,
-                                                          Taps t) {
+                                                          Taps t, const BlockDesc* __restrict__ bt) {
     static_assert(AXIS == 1 || AXIS == 2, "y or z");
+    if (bt) {   // blockIdx.z: the block, as in k_q_convx
+        const int64_t o = bt[blockIdx.z].off;
+        out0 += o;
+        out1 += o;
+        in0 += o;
+        in1 += o;
+    }
     constexpr int m = (P - 1) / 2, NV = R + P - 1;
     const int x = blockIdx.x * 64 + threadIdx.x;
     if (x >= w) return;
@@ -314,15 +339,18 @@ __global__ __launch_bounds__(256) void k_conv_same_zero2r(float* __restrict__ ou
 
 // the y and z passes: Bh, Ba (the x pass) -> t1, t2 -> Bh, Ba
 template <int P>
-int conv2_yz(float* Bh, float* Ba, float* t1, float* t2, int w, int h, int l, const Taps& taps, hipStream_t s) {
+int conv2_yz(float* Bh, float* Ba, float* t1, float* t2, int w, int h, int l, const Taps& taps, hipStream_t s,
+             const BlockDesc* bt = nullptr, int nb = 1) {
     constexpr int R = 8;
     const dim3 blk(64, 4);
-    const dim3 gy((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)((h + R - 1) / R) * l + 3) / 4, 65535));
+    const dim3 gy((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)((h + R - 1) / R) * l + 3) / 4, 65535),
+                  (unsigned)nb);
     hipLaunchKernelGGL((k_conv_same_zero2r<1, P, R>), gy, blk, 0, s, t1, t2, (const float*)Bh, (const float*)Ba, w, h,
-                       l, taps);
-    const dim3 gz((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)((l + R - 1) / R) * h + 3) / 4, 65535));
+                       l, taps, bt);
+    const dim3 gz((unsigned)((w + 63) / 64), (unsigned)std::min<int64_t>(((int64_t)((l + R - 1) / R) * h + 3) / 4, 65535),
+                  (unsigned)nb);
     hipLaunchKernelGGL((k_conv_same_zero2r<2, P, R>), gz, blk, 0, s, Bh, Ba, (const float*)t1, (const float*)t2, w, h,
-                       l, taps);
+                       l, taps, bt);
     RSF_HIP(hipGetLastError());
     return RSF_OK;
 }
@@ -339,11 +367,11 @@ int conv2_axes(float* Bh, float* Ba, const float* QH, const float* QA, float* t1
 // the co-states formed inside the x pass (k_q_convx), then the y and z passes
 template <int P>
 int q_conv2_axes(float* Bh, float* Ba, const QArgs& a, float* t1, float* t2, int w, int h, int l, const Taps& taps,
-                 hipStream_t s) {
+                 hipStream_t s, const BlockDesc* bt, int nb) {
     const dim3 blk(64, 4);
-    const dim3 grid((unsigned)std::min<int64_t>(((int64_t)h * l + 3) / 4, 65535));
-    hipLaunchKernelGGL((k_q_convx<P>), grid, blk, (size_t)8 * w * sizeof(float), s, Bh, Ba, a, w, h, l, taps);
-    return conv2_yz<P>(Bh, Ba, t1, t2, w, h, l, taps, s);
+    const dim3 grid((unsigned)std::min<int64_t>(((int64_t)h * l + 3) / 4, 65535), (unsigned)nb);
+    hipLaunchKernelGGL((k_q_convx<P>), grid, blk, (size_t)8 * w * sizeof(float), s, Bh, Ba, a, w, h, l, taps, bt);
+    return conv2_yz<P>(Bh, Ba, t1, t2, w, h, l, taps, s, bt, nb);
 }
 
 // G_a at every voxel: normalisation Jacobian applied to D^T(kbar) plus D^T(Lbar).
@@ -376,7 +404,7 @@ __global__ void k_stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, con
 }
 
 // Per-workgroup partial sums in a fixed order: a shuffle tree within each 64-lane
-// wave, then the waves; row blockIdx.x of partials.
+// wave, then the waves; row blockIdx.y * gridDim.x + blockIdx.x of partials.
 __device__ __forceinline__ void block_partials(float* __restrict__ partials, float s0, float s1, float s2, float s3) {
     for (int o = 32; o > 0; o >>= 1) {
         s0 += __shfl_down(s0, o, 64);
@@ -397,7 +425,7 @@ __device__ __forceinline__ void block_partials(float* __restrict__ partials, flo
     if (threadIdx.x < 4) {
         float v = red[threadIdx.x][0];
         for (int q = 1; q < kW; ++q) v += red[threadIdx.x][q];
-        partials[4 * blockIdx.x + threadIdx.x] = v;
+        partials[4 * (blockIdx.y * gridDim.x + blockIdx.x) + threadIdx.x] = v;
     }
 }
 
@@ -504,8 +532,21 @@ __device__ __forceinline__ void quad_coords(uint32_t base, int w, int h, int& x0
 __global__ __launch_bounds__(kT) void k_stencil_g4(float* __restrict__ Gx, float* __restrict__ Gy,
                                                    float* __restrict__ Gz, FwdStencil fwd,
                                                    const float* __restrict__ phi, const float* __restrict__ gout,
-                                                   int w, int h, int l, float dt, float nu, float mu, float eps) {
+                                                   int w, int h, int l, float dt, float nu, float mu, float eps,
+                                                   const BlockDesc* __restrict__ bt) {
 #pragma clang fp contract(off)
+    if (bt) {   // blockIdx.y: the block, as in k_q_convx
+        const int64_t o = bt[blockIdx.y].off;
+        Gx += o;
+        Gy += o;
+        Gz += o;
+        for (int a = 0; a < 3; ++a) {
+            fwd.n[a] += o;
+            fwd.g[a] += o;
+        }
+        phi += o;
+        gout += o;
+    }
     const uint32_t W = (uint32_t)w, plane = W * (uint32_t)h, n = plane * (uint32_t)l;
     auto kbar = [&](float p, float u) { return (nu * dt * dirac(p, eps) - mu * dt) * u; };
     auto lbar = [&](float u) { return mu * dt * u; };
@@ -581,8 +622,26 @@ __global__ __launch_bounds__(kT) void k_finish4(float* __restrict__ gin, float* 
                                                 const float* __restrict__ Bh, const float* __restrict__ Ba,
                                                 const void* __restrict__ img, int u8, const float* __restrict__ fo,
                                                 const float* __restrict__ fi, int w, int h, int l, float dt, float nu,
-                                                float mu, float eps, float l1, float l2) {
+                                                float mu, float eps, float l1, float l2, const BlockDesc* __restrict__ bt) {
 #pragma clang fp contract(off)
+    if (bt) {   // blockIdx.y: the block, as in k_q_convx; its partial rows follow the previous block's
+        const int64_t o = bt[blockIdx.y].off;
+        gin += o;
+        phi += o;
+        gout += o;
+        Gx += o;
+        Gy += o;
+        Gz += o;
+        for (int a = 0; a < 3; ++a) {
+            fwd.n[a] += o;
+            fwd.g[a] += o;
+        }
+        Bh += o;
+        Ba += o;
+        img = static_cast<const char*>(img) + o * (u8 ? 1 : 4);
+        fo += o;
+        fi += o;
+    }
     const uint32_t W = (uint32_t)w, plane = W * (uint32_t)h, n = plane * (uint32_t)l;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
     for (uint32_t k = blockIdx.x * kT + threadIdx.x; k < n / 4; k += gridDim.x * kT) {
@@ -705,17 +764,19 @@ int blur_transpose(float* out, const float* in, float* t1, float* t2, int w, int
 
 int q_blur_transpose2(float* Bh, float* Ba, float* QH, float* QA, float* t1, float* t2, const float* phi,
                       const float* gout, const void* img, int u8, const float* fo, const float* fi, const float* PH,
-                      const float* PHc, int w, int h, int l, const Derived& d, const rsf_params* p, hipStream_t s) {
+                      const float* PHc, int w, int h, int l, const Derived& d, const rsf_params* p, hipStream_t s,
+                      const BlockDesc* bt, int nb) {
     const QArgs a{phi, gout, fo, fi, PH, PHc, img, u8, d.m, p->dt, p->lambda1, p->lambda2, p->epsilon};
     // rows of up to 2048 voxels fit the fused kernel's 64 KB of LDS
     if (w <= 2048) {
         switch (d.P) {
-            case 7: return q_conv2_axes<7>(Bh, Ba, a, t1, t2, w, h, l, d.taps, s);
-            case 21: return q_conv2_axes<21>(Bh, Ba, a, t1, t2, w, h, l, d.taps, s);
-            case 35: return q_conv2_axes<35>(Bh, Ba, a, t1, t2, w, h, l, d.taps, s);
+            case 7: return q_conv2_axes<7>(Bh, Ba, a, t1, t2, w, h, l, d.taps, s, bt, nb);
+            case 21: return q_conv2_axes<21>(Bh, Ba, a, t1, t2, w, h, l, d.taps, s, bt, nb);
+            case 35: return q_conv2_axes<35>(Bh, Ba, a, t1, t2, w, h, l, d.taps, s, bt, nb);
             default: break;
         }
     }
+    if (bt) return set_error(RSF_ERR_UNSUPPORTED, "no batched co-state pass for %d taps", d.P);
     RSF_TRY(pointwise_q(QH, QA, phi, gout, img, u8, fo, fi, PH, PHc, w, h, l, d.m, p, s));
     return blur_transpose2(Bh, Ba, QH, QA, t1, t2, w, h, l, d, s);
 }
@@ -741,12 +802,18 @@ bool quads_fit(int w, int64_t n, std::initializer_list<const void*> ptrs) {
     return true;
 }
 
+bool batch_ok(int P, int w, int64_t n, const void* img, int u8, std::initializer_list<const void*> ptrs) {
+    return (P == 7 || P == 21 || P == 35) && w <= 2048 && reinterpret_cast<uintptr_t>(img) % (u8 ? 4 : 16) == 0 &&
+           quads_fit(w, n, ptrs);
+}
+
 int stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, const float* phi, const float* gout, int w, int h,
-              int l, const rsf_params* p, hipStream_t s) {
+              int l, const rsf_params* p, hipStream_t s, const BlockDesc* bt, int nb) {
     const int64_t n = (int64_t)w * h * l;
-    if (quads_fit(w, n, {Gx, Gy, Gz, fwd.n[0], fwd.n[1], fwd.n[2], fwd.g[0], fwd.g[1], fwd.g[2], phi, gout})) {
-        hipLaunchKernelGGL(k_stencil_g4, grid_for(n / 4), dim3(kT), 0, s, Gx, Gy, Gz, fwd, phi, gout, w, h, l, p->dt,
-                           p->nu, p->mu, p->epsilon);
+    if (bt || quads_fit(w, n, {Gx, Gy, Gz, fwd.n[0], fwd.n[1], fwd.n[2], fwd.g[0], fwd.g[1], fwd.g[2], phi, gout})) {
+        const dim3 grid(grid_for(n / 4).x, (unsigned)nb);
+        hipLaunchKernelGGL(k_stencil_g4, grid, dim3(kT), 0, s, Gx, Gy, Gz, fwd, phi, gout, w, h, l, p->dt, p->nu,
+                           p->mu, p->epsilon, bt);
     } else {
         hipLaunchKernelGGL(k_stencil_g, grid_for(n), dim3(kT), 0, s, Gx, Gy, Gz, fwd, phi, gout, w, h, l, p->dt,
                            p->nu, p->mu, p->epsilon);
@@ -764,14 +831,16 @@ int finish_rows(int64_t n) { return (int)grid_finish(n).x; }
 
 int finish(float* gin, float* partials, const float* phi, const float* gout, const float* Gx, const float* Gy,
            const float* Gz, FwdStencil fwd, const float* Bh, const float* Ba, const void* img, int u8,
-           const float* fo, const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s) {
+           const float* fo, const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s,
+           const BlockDesc* bt, int nb) {
     const int64_t n = (int64_t)w * h * l;
     // the same grid either way (finish_rows partial rows); a uint8 image is read 4 bytes at a time
     const bool img_ok = reinterpret_cast<uintptr_t>(img) % (u8 ? 4 : 16) == 0;
-    if (img_ok && quads_fit(w, n, {gin, phi, gout, Gx, Gy, Gz, fwd.n[0], fwd.n[1], fwd.n[2], fwd.g[0], fwd.g[1],
-                                   fwd.g[2], Bh, Ba, fo, fi})) {
-        hipLaunchKernelGGL(k_finish4, grid_finish(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, fwd, Bh,
-                           Ba, img, u8, fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2);
+    if (bt || (img_ok && quads_fit(w, n, {gin, phi, gout, Gx, Gy, Gz, fwd.n[0], fwd.n[1], fwd.n[2], fwd.g[0],
+                                          fwd.g[1], fwd.g[2], Bh, Ba, fo, fi}))) {
+        const dim3 grid(grid_finish(n).x, (unsigned)nb);
+        hipLaunchKernelGGL(k_finish4, grid, dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, fwd, Bh, Ba, img, u8,
+                           fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2, bt);
     } else {
         hipLaunchKernelGGL(k_finish, grid_finish(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, fwd, Bh,
                            Ba, img, u8, fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2);

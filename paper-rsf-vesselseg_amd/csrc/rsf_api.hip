@@ -1314,6 +1314,26 @@ int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks
     RSF_TRY(launch_sweep1(pr.runner->plan, a, pr.d, p, s));
     const size_t esz = u8 ? 1 : 4;
     int row = 0;
+    // blocks of one shape, each starting on a 4-voxel boundary: every kernel once
+    // for all of them (the same bits as block by block)
+    const BlockDesc& b0 = pr.blocks[0];
+    const int64_t n0 = (int64_t)b0.w * b0.h * b0.l;
+    bool batch = pr.blocks.size() > 1;
+    for (auto& b : pr.blocks) batch = batch && b.w == b0.w && b.h == b0.h && b.l == b0.l && b.off % 4 == 0;
+    if (batch && adj::batch_ok(pr.d.P, b0.w, n0, image, u8,
+                               {fo, fi, PH, PHc, QH, QA, Bh, Ba, Gx, Gy, Gz, t1, t2, phi, grad_out, grad_in})) {
+        const BlockDesc* bt = pr.runner->plan.d_blocks.as<BlockDesc>();
+        const int nb = (int)pr.blocks.size();
+        RSF_TRY(adj::q_blur_transpose2(Bh, Ba, QH, QA, t1, t2, phi, grad_out, image, u8, fo, fi, PH, PHc, b0.w, b0.h,
+                                       b0.l, pr.d, p, s, bt, nb));
+        const adj::FwdStencil fwd{{PH, PHc, QH}, {QA, t1, t2}};
+        RSF_TRY(adj::stencil_g(Gx, Gy, Gz, fwd, phi, grad_out, b0.w, b0.h, b0.l, p, s, bt, nb));
+        RSF_TRY(adj::finish(grad_in, parts, phi, grad_out, Gx, Gy, Gz, fwd, Bh, Ba, image, u8, fo, fi, b0.w, b0.h,
+                            b0.l, p, s, bt, nb));
+        RSF_TRY(adj::reduce_params(grad_params, parts, nb * adj::finish_rows(n0), s));
+        RSF_HIP(hipStreamSynchronize(s));
+        return RSF_OK;
+    }
     for (auto& b : pr.blocks) {
         const int64_t o = b.off;
         const void* im = (const char*)image + o * esz;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -192,10 +193,19 @@ int blur_transpose(float* out, const float* in, float* t1, float* t2, int w, int
 int blur_transpose2(float* Bh, float* Ba, const float* QH, const float* QA, float* t1, float* t2, int w,
                     int h, int l, const Derived& d, hipStream_t s);
 // pointwise_q then blur_transpose2, the co-states formed inside the x pass where
-// the tap count and the row length allow (QH, QA then untouched)
+// the tap count and the row length allow (QH, QA then untouched).
+//
+// q_blur_transpose2, stencil_g and finish also run nb blocks of one shape in one
+// launch each: bt is then the device block table (pointers are the buffers' starts,
+// each kernel adds its block's offset) and batch_ok(P, w, n, img, u8, {buffers})
+// must hold for the buffers and for every block's first voxel (4-voxel aligned).
+// The arithmetic and the partial rows (nb * finish_rows(n), block after block) are
+// those of the per-block calls.
 int q_blur_transpose2(float* Bh, float* Ba, float* QH, float* QA, float* t1, float* t2, const float* phi,
                       const float* gout, const void* img, int u8, const float* fo, const float* fi, const float* PH,
-                      const float* PHc, int w, int h, int l, const Derived& d, const rsf_params* p, hipStream_t s);
+                      const float* PHc, int w, int h, int l, const Derived& d, const rsf_params* p, hipStream_t s,
+                      const BlockDesc* bt = nullptr, int nb = 1);
+bool batch_ok(int P, int w, int64_t n, const void* img, int u8, std::initializer_list<const void*> ptrs);
 // The forward step's normal n = g/(|g|+1e-6) and gradient g at every voxel, which
 // stencil_g computes and finish reads at a voxel's neighbours (its curvature and
 // Laplacian).
@@ -204,11 +214,12 @@ struct FwdStencil {
     float* g[3];
 };
 int stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, const float* phi, const float* gout, int w,
-              int h, int l, const rsf_params* p, hipStream_t s);
+              int h, int l, const rsf_params* p, hipStream_t s, const BlockDesc* bt = nullptr, int nb = 1);
 int finish_rows(int64_t n);
 int finish(float* gin, float* partials, const float* phi, const float* gout, const float* Gx,
            const float* Gy, const float* Gz, FwdStencil fwd, const float* Bh, const float* Ba, const void* img,
-           int u8, const float* fo, const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s);
+           int u8, const float* fo, const float* fi, int w, int h, int l, const rsf_params* p, hipStream_t s,
+           const BlockDesc* bt = nullptr, int nb = 1);
 int reduce_params(float* gparams, const float* partials, int rows, hipStream_t s);
 }  // namespace adj
 

@@ -79,6 +79,38 @@ def test_adjoint_blocks_equal_single(ctx):
     np.testing.assert_allclose(gp.cpu().numpy(), gp_sum, rtol=1e-5)
 
 
+@pytest.mark.parametrize("u8", [True, False])
+def test_adjoint_uniform_batch_equals_single(ctx, u8):
+    """Blocks of one shape on 4-voxel boundaries run every adjoint kernel once for
+    the whole batch (block index in the grid): the same bits as each block alone."""
+    import torch
+
+    shape = (36, 34, 40)
+    blocks, total = rsf_amd.pack_blocks([shape] * 3)
+    img = np.empty(total, np.uint8)
+    phi = np.empty(total, np.float32)
+    for k, (off, w, h, l) in enumerate(blocks):
+        bi, bp, _ = scene(shape, tubes=4, seed=60 + k)
+        img[off:off + bi.size] = bi.ravel()
+        phi[off:off + bp.size] = bp.ravel()
+    u = np.random.default_rng(9).normal(size=total).astype(np.float32)
+    p = _params(3.0, nu=30.0)
+    conv = (lambda a: a) if u8 else (lambda a: a.astype(np.float32))
+    gin = torch.empty(total, device="cuda")
+    gp = torch.zeros(4, device="cuda")
+    ctx.adjoint(blocks, to_dev(conv(img)), to_dev(phi), to_dev(u), gin, gp, p)
+    gp_sum = np.zeros(4)
+    for (off, w, h, l) in blocks:
+        n = w * h * l
+        g1 = torch.empty(n, device="cuda")
+        p1 = torch.zeros(4, device="cuda")
+        ctx.adjoint([(0, w, h, l)], to_dev(conv(img[off:off + n])), to_dev(phi[off:off + n]), to_dev(u[off:off + n]),
+                    g1, p1, p)
+        assert torch.equal(g1, gin[off:off + n])
+        gp_sum += p1.cpu().numpy()
+    np.testing.assert_allclose(gp.cpu().numpy(), gp_sum, rtol=1e-5)
+
+
 @pytest.mark.parametrize("shape,u8", [((36, 40, 44), True), ((32, 30, 48), False), ((40, 36, 128), True)])
 def test_adjoint_quad_kernels_equal_scalar(ctx, shape, u8):
     """Rows a multiple of 4 long on 16-byte aligned buffers run the stencil and
