@@ -289,17 +289,40 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
             }
         }
     }
-    // a strip xy kernel wastes the lanes past a block's valid columns: blocks
-    // narrower than three strips take the first tiled variant
+    // a strip xy kernel wastes the lanes past a block's valid columns and runs one
+    // wave per strip and plane: where some block is narrower than three strips, the
+    // first strip variant of half the width (same z work items) takes over if it
+    // wastes 15 % fewer lanes, or if the wider strips give under 8 waves per CU
+    // (measured: 96^3 literal 0.092 -> 0.082 ms, 160^3 0.145 -> 0.130; fast 128^3
+    // 0.115 -> 0.098, 8 x 128^3 0.169 -> 0.151; 8 x 128^3 literal keeps the 64-column
+    // strips, 0.222 against 0.235)
     if (*s1 >= 0 && sweep1_variants()[*s1].strip > 0) {
+        const Sweep1Variant& cur = sweep1_variants()[*s1];
         bool narrow = false;
-        for (const auto& b : blocks) narrow = narrow || b.w - 2 * d.m < 3 * sweep1_variants()[*s1].tx;
+        for (const auto& b : blocks) narrow = narrow || b.w - 2 * d.m < 3 * cur.tx;
         for (size_t k = 0; narrow && k < sweep1_variants().size(); ++k) {
             const auto& v = sweep1_variants()[k];
-            if (v.P == d.P && v.literal == literal && v.strip == 0) {
-                *s1 = (int)k;
-                break;
-            }
+            if (v.P != d.P || v.literal != literal || v.strip == 0 || 2 * v.tx != cur.tx ||
+                v.tx2 * v.ty2 != cur.tx2 * cur.ty2)
+                continue;
+            // valid columns over computed columns, planes weighted by their rows
+            auto waste = [&](int tx, double* eff, int64_t* waves) {
+                double valid = 0, computed = 0;
+                *waves = 0;
+                for (const auto& b : blocks) {
+                    const int vw = std::max(b.w - 2 * d.m, 1), strips = (vw + tx - 1) / tx;
+                    valid += (double)vw * b.h * b.l;
+                    computed += (double)strips * tx * b.h * b.l;
+                    *waves += (int64_t)strips * b.l;
+                }
+                *eff = valid / computed;
+            };
+            double e_cur, e_alt;
+            int64_t w_cur, w_alt;
+            waste(cur.tx, &e_cur, &w_cur);
+            waste(v.tx, &e_alt, &w_alt);
+            if (e_alt >= 1.15 * e_cur || w_cur < 8 * (int64_t)c->num_cus) *s1 = (int)k;
+            break;
         }
     }
     const int lmax = max_dim_l(blocks);
