@@ -347,9 +347,17 @@ void pick_variants(rsf_context* c, const std::vector<BlockDesc>& blocks, const D
         return v.strip > 0 ? std::min(lmax, v.strip) : std::min(lmax, vox >= (int64_t(1) << 28) ? 128 : 32);
     };
     if (*s1 >= 0 && sweep1_variants()[*s1].fn2) *zc1 = split_zc1(sweep1_variants()[*s1]);
-    *zc2 = *s2 >= 0 ? std::min(chunk(sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, sweep2_variants()[*s2].ty,
-                                     4, 16), 64)
-                    : std::min(lmax, 64);
+    // sweep 2: chunks of at least 16 planes, down to 4 where 16 would leave CUs without
+    // a work item (a single 128^3 block: LITERAL 0.050 -> 0.030 ms, FAST 0.034 -> 0.020)
+    if (*s2 >= 0) {
+        const int tx2 = sweep2_variants()[*s2].tx * sweep2_variants()[*s2].vec, ty2 = sweep2_variants()[*s2].ty;
+        int64_t cols = 0;
+        for (const auto& b : blocks) cols += (int64_t)((b.w + tx2 - 1) / tx2) * ((b.h + ty2 - 1) / ty2);
+        const int zmin = cols * ((lmax + 15) / 16) < (int64_t)c->num_cus ? 4 : 16;
+        *zc2 = std::min(chunk(tx2, ty2, 4, zmin), 64);
+    } else {
+        *zc2 = std::min(lmax, 64);
+    }
     // explicit overrides: RSF_SWEEP1 / RSF_SWEEP2 = "<variant name>:<z chunk>"
     for (int which = 1; which <= 2; ++which) {
         const char* env = std::getenv(which == 1 ? "RSF_SWEEP1" : "RSF_SWEEP2");
