@@ -2508,7 +2508,7 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         // fast default: the strip xy kernel, 128-column strips, a plane per wave (xy
         // kernel at 512^3: 0.49 ms against 0.60 for the 64x44 tiles below)
         s1strip<21, false, 4, 4, 1, 4, 256, 3>("p21_fast_strip_128w4"),
-        // the fast default for blocks narrower than three strips
+        // tiled xy kernels on channel planes (tuner candidates)
         s1s<21, false, 64, 44, 512, 11, 4, 256, 1, 3>("p21_fast_split_64x44w4"),
         s1s<21, false, 64, 28, 512, 7, 4, 256, 1, 3>("p21_fast_split_64x28w4"),
         s1s<21, false, 32, 28, 256, 7, 3, 256, 1, 3>("p21_fast_split_32x28w3"),
@@ -2516,7 +2516,8 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         // x and y passes on the matrix cores: a tuner candidate, 1.56 ms per sweep 1 at
         // 512^3 against 1.36 for the default (MFMA at about a third of its rate)
         s1m<21, 64, 256, 2, 256, 1, 3>("p21_fast_mfma_64x28"),
-        // the strip xy kernel in fast mode, 64-column strips
+        // the strip xy kernel in fast mode, 64-column strips (the first of them the fast
+        // default for blocks narrower than three 128-column strips)
         s1strip<21, false, 2, 4, 3, 5, 256, 3>("p21_fast_strip_64w5"),
         s1strip<21, false, 2, 4, 3, 4, 256, 3>("p21_fast_strip_64w4"),
         // literal default: the strip xy kernel, 64-column strips, a plane per wave (xy
@@ -2525,13 +2526,15 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         // workgroup per CU (0.520 ms against 0.543 with 256 columns at 512^3: a
         // workgroup's reads of a plane span 16 KB instead of 4)
         s1strip<21, true, 4, 4, 1, 4, 1024, 3>("p21_lit_strip_64z1k"),
-        // the same with 256-column z work items: the literal default for planes that do
-        // not give every CU a 1024-column work item
+        // 256-column z work items and 3 rows of loads in flight: the literal default for
+        // planes that do not give every CU a 1024-column work item (sweep 1 at 256^3:
+        // 0.236 ms against 0.281 with one row in flight, 96^3 to 160^3 and 8 x 128^3
+        // 3-7% faster)
+        s1strip<21, true, 4, 4, 3, 4, 256, 3>("p21_lit_strip_64w4d3"),
         s1strip<21, true, 4, 4, 1, 4, 256, 3>("p21_lit_strip_64w4"),
-        // the literal default for blocks narrower than three strips (64 uneven blocks of
-        // sides 96 to 256: 3% faster than the strips): 32x60 tiles, 15 outputs per y
-        // item, each x-pass item forming all four channels from one H/I window, the
-        // intermediate interleaved per voxel (sweep 1 at 512^3: 2.05 ms before the z
+        // 32x60 tiles, 15 outputs per y item, each x-pass item forming all four channels
+        // from one H/I window, the intermediate interleaved per voxel (the default for
+        // narrow blocks before the 32-column strips; sweep 1 at 512^3: 2.05 ms before the z
         // kernel's band rework; 2.20 with channel planes, 2.24 with channel pairs per
         // x-pass item)
         s1s<21, true, 32, 60, 256, 15, 2, 256, 1, 3, 2, 4, 1, 1, true>("p21_lit_il_32x60r15"),
@@ -2548,9 +2551,8 @@ const std::vector<Sweep1Variant>& sweep1_variants() {
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true>("p21_lit_hi_32x44w2"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, true, 8>("p21_lit_hi_32x44x8"),
         s1s<21, true, 32, 44, 256, 11, 2, 256, 1, 3, 2, 8>("p21_lit_q4_32x44x8"),
-        // more strip kernels: 3 rows of loads in flight, 3 waves per SIMD, 32-column
-        // strips (xy kernel at 512^3: 0.82, 0.83, 0.98 and 1.01 ms)
-        s1strip<21, true, 4, 4, 3, 4, 256, 3>("p21_lit_strip_64w4d3"),
+        // more strip kernels: 3 waves per SIMD, 32-column strips (the first 32-column one
+        // the literal default for narrow blocks; xy kernel at 512^3: 0.83, 0.98 and 1.01 ms)
         s1strip<21, true, 4, 4, 3, 3, 256, 3>("p21_lit_strip_64w3"),
         s1strip<21, true, 2, 4, 3, 7, 256, 3>("p21_lit_strip_32w7"),
         s1strip<21, true, 2, 4, 1, 8, 256, 3>("p21_lit_strip_32w8"),
