@@ -24,14 +24,17 @@ class _Unrolled(torch.autograd.Function):
         th = [float(v) for v in theta.detach().cpu().tolist()]
         p = make_params(lambda1=th[0], lambda2=th[1], nu=th[2], mu=th[3], iterations=1, **fixed)
         cur = phi0.detach().contiguous().clone()
-        nxt = torch.empty_like(cur)
         ckpts = [cur.clone()]
         rsf.hold_image(image)   # the image is constant over the unrolled steps
         try:
-            for t in range(K):
-                rsf.step_blocks(blocks, image, cur, nxt, p)
-                cur, nxt = nxt, cur
-                if (t + 1) % checkpoint_every == 0 and t + 1 < K:
+            # one graph-replayed run per checkpoint segment, in place: the same kernels
+            # as step by step (the backward recomputes the segments with step_blocks),
+            # without a transform and a host round trip per step
+            for s in range(0, K, checkpoint_every):
+                n = min(checkpoint_every, K - s)
+                seg = make_params(lambda1=th[0], lambda2=th[1], nu=th[2], mu=th[3], iterations=n, **fixed)
+                rsf.run_batched(blocks, image, cur, seg)
+                if s + n < K:
                     ckpts.append(cur.clone())
         finally:
             rsf.release_image()

@@ -294,3 +294,33 @@ def test_held_image_gives_identical_results(ctx):
     torch.cuda.synchronize()
     fresh.close()
     assert np.array_equal(b1.cpu().numpy(), b2.cpu().numpy())
+
+
+@pytest.mark.parametrize("mode", [rsf_amd.MODE_FAST, rsf_amd.MODE_LITERAL])
+def test_unrolled_forward_equals_steps(ctx, mode):
+    """The unrolled forward runs each checkpoint segment as one graph-replayed
+    batched run; the backward recomputes the segments step by step.  Both give
+    the same bits: K = 11 with checkpoints every 4 (a short last segment)."""
+    import torch
+
+    from rsf_amd.autograd import rsf_unrolled
+
+    shapes = [(30, 32, 34), (28, 30, 36)]
+    blocks, total = rsf_amd.pack_blocks(shapes)
+    img = np.empty(total, np.uint8)
+    phi = np.empty(total, np.float32)
+    for k, ((off, w, h, l), shp) in enumerate(zip(blocks, shapes)):
+        bi, bp, _ = scene(shp, tubes=4, seed=80 + k)
+        img[off:off + bi.size] = bi.ravel()
+        phi[off:off + bp.size] = bp.ravel()
+    K, C = 11, 4
+    d_img = to_dev(img)
+    theta = torch.tensor([1.0, 1.0, 65.025, 1.0], device="cuda")
+    out = rsf_unrolled(to_dev(phi), theta, d_img, blocks, K, C, sigma=3.0, epsilon=1.0, mode=mode, context=ctx)
+    p = _params(3.0, mode=mode)
+    a, b = to_dev(phi), torch.empty(total, device="cuda")
+    for _ in range(K):
+        ctx.step_blocks(blocks, d_img, a, b, p)
+        a, b = b, a
+    torch.cuda.synchronize()
+    assert np.array_equal(out.detach().cpu().numpy(), a.cpu().numpy())
