@@ -42,6 +42,10 @@ CASES = [
     (1.0, 1.0, rsf_amd.MODE_FAST, np.float32, (30, 33, 36)),
     (5.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (40, 41, 44)),
     (3.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (23, 22, 24)),  # one row of quads per tile
+    # P = 21 blocks a few columns wider than the blur: the narrow-block strip kernels
+    (3.0, 1.0, rsf_amd.MODE_LITERAL, np.uint8, (23, 22, 24)),
+    (3.0, 1.0, rsf_amd.MODE_LITERAL, np.float32, (26, 23, 25)),
+    (3.0, 1.0, rsf_amd.MODE_FAST, np.uint8, (26, 23, 25)),
 ]
 
 
