@@ -542,7 +542,7 @@ __global__ __launch_bounds__(kT) void k_stencil_g4(float* __restrict__ Gx, float
         Gz += o;
         for (int a = 0; a < 3; ++a) {
             fwd.n[a] += o;
-            fwd.g[a] += o;
+            if (fwd.g[a]) fwd.g[a] += o;
         }
         phi += o;
         gout += o;
@@ -609,9 +609,11 @@ __global__ __launch_bounds__(kT) void k_stencil_g4(float* __restrict__ Gx, float
         st4(fwd.n[0], base, n0);
         st4(fwd.n[1], base, n1);
         st4(fwd.n[2], base, n2);
-        st4(fwd.g[0], base, g0);
-        st4(fwd.g[1], base, g1);
-        st4(fwd.g[2], base, g2);
+        if (fwd.g[0]) {   // the one-voxel finishing kernel reads them; the quad one forms them from phi
+            st4(fwd.g[0], base, g0);
+            st4(fwd.g[1], base, g1);
+            st4(fwd.g[2], base, g2);
+        }
     }
 }
 
@@ -632,10 +634,7 @@ __global__ __launch_bounds__(kT) void k_finish4(float* __restrict__ gin, float* 
         Gx += o;
         Gy += o;
         Gz += o;
-        for (int a = 0; a < 3; ++a) {
-            fwd.n[a] += o;
-            fwd.g[a] += o;
-        }
+        for (int a = 0; a < 3; ++a) fwd.n[a] += o;
         Bh += o;
         Ba += o;
         img = static_cast<const char*>(img) + o * (u8 ? 1 : 4);
@@ -650,13 +649,12 @@ __global__ __launch_bounds__(kT) void k_finish4(float* __restrict__ gin, float* 
         quad_coords(base, w, h, x0, y, z);
         const uint32_t iym = y > 0 ? base - W : base, iyp = y + 1 < h ? base + W : base;
         const uint32_t izm = z > 0 ? base - plane : base, izp = z + 1 < l ? base + plane : base;
-        // G along x, y, z; the forward normals and gradients: component x along the
-        // row, y at the rows y+-1, z at the planes z+-1
-        float gxr[6], n0r[6], g0r[6], gy0[4], gym[4], gyp[4], gz0[4], gzm[4], gzp[4];
-        float n1m[4], n1p[4], g1m[4], g1p[4], n2m[4], n2p[4], g2m[4], g2p[4];
+        // G along x, y, z; the forward normals: component x along the row, y at the
+        // rows y+-1, z at the planes z+-1
+        float gxr[6], n0r[6], gy0[4], gym[4], gyp[4], gz0[4], gzm[4], gzp[4];
+        float n1m[4], n1p[4], n2m[4], n2p[4];
         ld6(gxr, Gx, base, n);
         ld6(n0r, fwd.n[0], base, n);
-        ld6(g0r, fwd.g[0], base, n);
         ld4(gy0, Gy, base);
         ld4(gym, Gy, iym);
         ld4(gyp, Gy, iyp);
@@ -665,12 +663,25 @@ __global__ __launch_bounds__(kT) void k_finish4(float* __restrict__ gin, float* 
         ld4(gzp, Gz, izp);
         ld4(n1m, fwd.n[1], iym);
         ld4(n1p, fwd.n[1], iyp);
-        ld4(g1m, fwd.g[1], iym);
-        ld4(g1p, fwd.g[1], iyp);
         ld4(n2m, fwd.n[2], izm);
         ld4(n2p, fwd.n[2], izp);
-        ld4(g2m, fwd.g[2], izm);
-        ld4(g2p, fwd.g[2], izp);
+        // the forward gradient's components at the Laplacian's neighbours, from phi
+        // as grad_at forms them: along x at the linear neighbours (phi at base-2 ..
+        // base+5, clamped to the block), along y and z from the rows and planes within
+        // 2 (clamped: the one-sided rules at the faces)
+        float fr[8], pym2[4], pyp2[4], pzm2[4], pzp2[4], pym1[4], pyp1[4], pzm1[4], pzp1[4];
+        fr[0] = phi[base >= 2 ? base - 2 : 0];
+        fr[1] = phi[base >= 1 ? base - 1 : 0];
+        fr[6] = phi[base + 4 < n ? base + 4 : n - 1];
+        fr[7] = phi[base + 5 < n ? base + 5 : n - 1];
+        ld4(pym1, phi, iym);
+        ld4(pyp1, phi, iyp);
+        ld4(pzm1, phi, izm);
+        ld4(pzp1, phi, izp);
+        ld4(pym2, phi, y > 1 ? base - 2 * W : (y > 0 ? base - W : base));
+        ld4(pyp2, phi, y + 2 < h ? base + 2 * W : (y + 1 < h ? base + W : base));
+        ld4(pzm2, phi, z > 1 ? base - 2 * plane : (z > 0 ? base - plane : base));
+        ld4(pzp2, phi, z + 2 < l ? base + 2 * plane : (z + 1 < l ? base + plane : base));
         float I[4], fo4[4], fi4[4], p4[4], u4[4], bh[4], ba[4];
         if (u8) {
             const uint32_t v = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(img) + base);
@@ -685,6 +696,18 @@ __global__ __launch_bounds__(kT) void k_finish4(float* __restrict__ gin, float* 
         ld4(u4, gout, base);
         ld4(bh, Bh, base);
         ld4(ba, Ba, base);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fr[i + 2] = p4[i];
+        // g_x at the linear element base-1+s: its right neighbour is itself at x = w-1
+        // (base-1 at x0 = 0 is the previous row's last voxel), its left one the
+        // previous element
+        float g0v[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const int xq = x0 - 1 + s;
+            const bool self = xq == w - 1 || xq == -1;
+            g0v[s] = ((self ? fr[s + 1] : fr[s + 2]) - fr[s]) * 0.5f;
+        }
         float o[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -694,8 +717,12 @@ __global__ __launch_bounds__(kT) void k_finish4(float* __restrict__ gin, float* 
                              dTyz_v(gzm[i], gz0[i], gzp[i], z, l);
             const float kap = (((x == w - 1 ? n0r[i + 1] : n0r[i + 2]) - n0r[i]) + (n1p[i] - n1m[i]) +
                                (n2p[i] - n2m[i])) * 0.5f;
-            const float lap = (((x == w - 1 ? g0r[i + 1] : g0r[i + 2]) - g0r[i]) + (g1p[i] - g1m[i]) +
-                               (g2p[i] - g2m[i])) * 0.5f;
+            const float g0R = x == w - 1 ? g0v[i + 1] : g0v[i + 2], g0L = j == 0 ? g0v[1] : g0v[i];
+            const float g1R = y == h - 1 ? (p4[i] - pym1[i]) * 0.5f : (pyp2[i] - p4[i]) * 0.5f;
+            const float g1L = y == 0 ? (pyp1[i] - p4[i]) * 0.5f : (p4[i] - pym2[i]) * 0.5f;
+            const float g2R = z == l - 1 ? (p4[i] - pzm1[i]) * 0.5f : (pzp2[i] - p4[i]) * 0.5f;
+            const float g2L = z == 0 ? (pzp1[i] - p4[i]) * 0.5f : (p4[i] - pzm2[i]) * 0.5f;
+            const float lap = ((g0R - g0L) + (g1R - g1L) + (g2R - g2L)) * 0.5f;
             const float d1 = I[i] - fo4[i], d2 = I[i] - fi4[i];
             const float Eo = d1 * d1, Ei = d2 * d2;
             const float F = l1 * Eo - l2 * Ei;
@@ -802,9 +829,12 @@ bool quads_fit(int w, int64_t n, std::initializer_list<const void*> ptrs) {
     return true;
 }
 
+bool quads_ok(int w, int64_t n, const void* img, int u8, std::initializer_list<const void*> ptrs) {
+    return reinterpret_cast<uintptr_t>(img) % (u8 ? 4 : 16) == 0 && quads_fit(w, n, ptrs);
+}
+
 bool batch_ok(int P, int w, int64_t n, const void* img, int u8, std::initializer_list<const void*> ptrs) {
-    return (P == 7 || P == 21 || P == 35) && w <= 2048 && reinterpret_cast<uintptr_t>(img) % (u8 ? 4 : 16) == 0 &&
-           quads_fit(w, n, ptrs);
+    return (P == 7 || P == 21 || P == 35) && w <= 2048 && quads_ok(w, n, img, u8, ptrs);
 }
 
 int stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, const float* phi, const float* gout, int w, int h,
@@ -815,6 +845,7 @@ int stencil_g(float* Gx, float* Gy, float* Gz, FwdStencil fwd, const float* phi,
         hipLaunchKernelGGL(k_stencil_g4, grid, dim3(kT), 0, s, Gx, Gy, Gz, fwd, phi, gout, w, h, l, p->dt, p->nu,
                            p->mu, p->epsilon, bt);
     } else {
+        if (!fwd.g[0]) return set_error(RSF_ERR_INVALID_ARGUMENT, "the one-voxel stencil kernel stores the gradients");
         hipLaunchKernelGGL(k_stencil_g, grid_for(n), dim3(kT), 0, s, Gx, Gy, Gz, fwd, phi, gout, w, h, l, p->dt,
                            p->nu, p->mu, p->epsilon);
     }
@@ -836,12 +867,13 @@ int finish(float* gin, float* partials, const float* phi, const float* gout, con
     const int64_t n = (int64_t)w * h * l;
     // the same grid either way (finish_rows partial rows); a uint8 image is read 4 bytes at a time
     const bool img_ok = reinterpret_cast<uintptr_t>(img) % (u8 ? 4 : 16) == 0;
-    if (bt || (img_ok && quads_fit(w, n, {gin, phi, gout, Gx, Gy, Gz, fwd.n[0], fwd.n[1], fwd.n[2], fwd.g[0],
-                                          fwd.g[1], fwd.g[2], Bh, Ba, fo, fi}))) {
+    if (bt || (img_ok && quads_fit(w, n, {gin, phi, gout, Gx, Gy, Gz, fwd.n[0], fwd.n[1], fwd.n[2], Bh, Ba, fo,
+                                          fi}))) {
         const dim3 grid(grid_finish(n).x, (unsigned)nb);
         hipLaunchKernelGGL(k_finish4, grid, dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, fwd, Bh, Ba, img, u8,
                            fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2, bt);
     } else {
+        if (!fwd.g[0]) return set_error(RSF_ERR_INVALID_ARGUMENT, "the one-voxel finishing kernel reads the gradients");
         hipLaunchKernelGGL(k_finish, grid_finish(n), dim3(kT), 0, s, gin, partials, phi, gout, Gx, Gy, Gz, fwd, Bh,
                            Ba, img, u8, fo, fi, w, h, l, p->dt, p->nu, p->mu, p->epsilon, p->lambda1, p->lambda2);
     }

@@ -1357,7 +1357,8 @@ int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks
         const int nb = (int)pr.blocks.size();
         RSF_TRY(adj::q_blur_transpose2(Bh, Ba, QH, QA, t1, t2, phi, grad_out, image, u8, fo, fi, PH, PHc, b0.w, b0.h,
                                        b0.l, pr.d, p, s, bt, nb));
-        const adj::FwdStencil fwd{{PH, PHc, QH}, {QA, t1, t2}};
+        // the quad finishing kernel forms the forward gradients from phi: none stored
+        const adj::FwdStencil fwd{{PH, PHc, QH}, {nullptr, nullptr, nullptr}};
         RSF_TRY(adj::stencil_g(Gx, Gy, Gz, fwd, phi, grad_out, b0.w, b0.h, b0.l, p, s, bt, nb));
         RSF_TRY(adj::finish(grad_in, parts, phi, grad_out, Gx, Gy, Gz, fwd, Bh, Ba, image, u8, fo, fi, b0.w, b0.h,
                             b0.l, p, s, bt, nb));
@@ -1371,8 +1372,14 @@ int rsf_step_adjoint_device(rsf_context* c, const rsf_block* blocks, int nblocks
         RSF_TRY(adj::q_blur_transpose2(Bh + o, Ba + o, QH + o, QA + o, t1 + o, t2 + o, phi + o, grad_out + o, im, u8,
                                        fo + o, fi + o, PH + o, PHc + o, b.w, b.h, b.l, pr.d, p, s));
         // the forward normals and gradients into the buffers free by now (PH, PHc:
-        // read by the point-wise step; QH, QA, t1, t2: by the transposed blur)
-        const adj::FwdStencil fwd{{PH + o, PHc + o, QH + o}, {QA + o, t1 + o, t2 + o}};
+        // read by the point-wise step; QH, QA, t1, t2: by the transposed blur); the
+        // gradients only for the one-voxel kernels, where some buffer is not aligned
+        // for the quad ones
+        const bool quads = adj::quads_ok(b.w, (int64_t)b.w * b.h * b.l, im, u8,
+                                         {Gx + o, Gy + o, Gz + o, PH + o, PHc + o, QH + o, phi + o, grad_out + o,
+                                          grad_in + o, Bh + o, Ba + o, fo + o, fi + o});
+        const adj::FwdStencil fwd = quads ? adj::FwdStencil{{PH + o, PHc + o, QH + o}, {nullptr, nullptr, nullptr}}
+                                          : adj::FwdStencil{{PH + o, PHc + o, QH + o}, {QA + o, t1 + o, t2 + o}};
         RSF_TRY(adj::stencil_g(Gx + o, Gy + o, Gz + o, fwd, phi + o, grad_out + o, b.w, b.h, b.l, p, s));
         RSF_TRY(adj::finish(grad_in + o, parts + 4 * row, phi + o, grad_out + o, Gx + o, Gy + o, Gz + o, fwd, Bh + o,
                             Ba + o, im, u8, fo + o, fi + o, b.w, b.h, b.l, p, s));

@@ -206,9 +206,13 @@ int q_blur_transpose2(float* Bh, float* Ba, float* QH, float* QA, float* t1, flo
                       const float* PHc, int w, int h, int l, const Derived& d, const rsf_params* p, hipStream_t s,
                       const BlockDesc* bt = nullptr, int nb = 1);
 bool batch_ok(int P, int w, int64_t n, const void* img, int u8, std::initializer_list<const void*> ptrs);
+// whether stencil_g and finish take their four-voxel kernels for a block (rows a
+// multiple of 4, every buffer 16-byte aligned, a uint8 image 4-byte aligned)
+bool quads_ok(int w, int64_t n, const void* img, int u8, std::initializer_list<const void*> ptrs);
 // The forward step's normal n = g/(|g|+1e-6) and gradient g at every voxel, which
 // stencil_g computes and finish reads at a voxel's neighbours (its curvature and
-// Laplacian).
+// Laplacian).  g may be null where both take their four-voxel kernels (quads_ok):
+// that finishing kernel forms the gradients from phi.
 struct FwdStencil {
     float* n[3];
     float* g[3];
